@@ -1,0 +1,164 @@
+/*
+ * spilady_hip.h - C ABI of libspilady_hip.so, the MI355X (gfx950) spin-lattice
+ * dynamics engine for the SLDHL Hamiltonian: DD05 magnetic EAM plus Jij
+ * Heisenberg exchange and Landau terms, FP64 throughout.
+ *
+ * The reference's operator API for this path is a set of argument-less,
+ * global-state C++ functions selected at link time between *_CPU.cpp and
+ * *_GPU.cu (prototype_CPU.h:120-289). Each entry point below replaces one of
+ * them (cited per function); the state they shared through globals
+ * (global.h:53-214) lives in an opaque context instead. The reference's
+ * compile-time switches (control.h:8-43) are runtime fields of sld_params.
+ *
+ * Conventions:
+ *  - every function returns 0 (SLD_OK) or a positive SLD_ERR_* code; the text
+ *    of the last error is returned by sld_last_error(ctx);
+ *  - host buffers are caller-owned and copied; device memory is owned by the
+ *    context; vectors are [natom][3] row-major, stress is [natom][6] in the
+ *    con- column order 11 22 33 12 23 31;
+ *  - calls on one context are ordered on that context's HIP stream and are
+ *    not re-entrant; one context per host thread.
+ */
+#ifndef SPILADY_HIP_H
+#define SPILADY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLD_ABI_VERSION 1
+
+#define SLD_OK 0
+#define SLD_ERR_ARG 1       /* invalid argument or input state */
+#define SLD_ERR_HIP 2       /* a HIP runtime call failed */
+#define SLD_ERR_CAPACITY 3  /* a fixed per-cell/per-atom capacity was exceeded */
+#define SLD_ERR_STATE 4     /* call not valid in the context's current state */
+#define SLD_ERR_NOGPU 5     /* no usable gfx950 device */
+
+/* Run parameters: the variables.in keys of read_variables.cpp:51-242 in
+ * internal units (default.h:44-226), plus the control.h switches. */
+typedef struct sld_params {
+    int32_t natom;
+    int32_t interval_of_print_out;  /* renormalizeEnergy acts on printed steps */
+    double box[6];             /* d.xx d.yx d.yy d.zx d.zy d.zz (read_config.cpp:41-42) */
+    double step;               /* time step, s */
+    double temperature;        /* k_B*T, eV (read_variables.cpp:159) */
+    double atmass;             /* eV/(A/s)^2 (read_variables.cpp:155-157) */
+    double gamma_L_over_mass;  /* 1/s */
+    double gamma_S_HL;         /* spin damping for the Heisenberg-Landau case */
+    double kappa_e;            /* electron thermal conductivity, W/(m K) */
+    double rcut_pot, rcut_mag, rcut_max;  /* A */
+    double min_length_link_cell;          /* A */
+    double start_time;         /* s */
+    uint64_t random_seed;
+    int32_t replica;           /* noise stream id (independent replicas) */
+    int32_t spinlang;          /* control.h: spin Langevin thermostat */
+    int32_t lattlang;          /* control.h: lattice Langevin thermostat */
+    int32_t eltemp;            /* control.h: per-cell electron temperature */
+    int32_t localcolmot;       /* control.h: local collective motion (needs eltemp) */
+    int32_t renormalize_energy;/* control.h: renormalizeEnergy (needs eltemp) */
+    int32_t reserved[8];
+} sld_params;
+
+/* Host view of the per-atom state (struct.h:44-120, con- columns of
+ * write_config.cpp:92-131). Any pointer may be NULL in sld_download to skip
+ * that field; r, p and s are required by sld_create. */
+typedef struct sld_atoms {
+    double *r, *p, *f, *s;
+    double *heff_h, *heff_l;
+    double *rho, *pe, *ke, *me, *me0;
+    double *stress;            /* [natom][6] */
+    double *vir, *sum_jij_sj;
+} sld_atoms;
+
+/* One printed step: the enr-, tmp- and spn- columns
+ * (check_energy_CPU.cpp:121-133, check_temperature_CPU.cpp:144-158,
+ * check_spin_CPU.cpp:66-76) plus the virial. */
+typedef struct sld_obs {
+    int32_t step;
+    int32_t pad;
+    double total_time;
+    double ave_pe, ave_ke, ave_me, ave_me0, ave_ee;
+    double tl, ts_r, ts_l, te;
+    double ave_s[3], ave_s0, ave_m[3], ave_m0;
+    double virial;
+} sld_obs;
+
+/* Per-cell electron subsystem (struct.h:122-166); any pointer may be NULL. */
+typedef struct sld_cells {
+    double *te, *ee, *tl, *ts_r, *ts_l, *gel, *ges;
+    double *sum_r_up, *sum_r_dn, *sum_l_up, *sum_l_dn;
+} sld_cells;
+
+typedef struct sld_ctx sld_ctx;
+
+/* Library identification: ABI version and the device it bound to. */
+int sld_version(void);
+int sld_device_info(char* name, int len, int* n_cu);
+
+/* Replaces initial_GPU + potential_table_GPU + copy_CPU_to_GPU +
+ * initial_links + allocate_cells + initialize_random_number and, with eltemp,
+ * the map_cells Te set-up and the initial calculate_temperature
+ * (initial.cpp:31-106, initial_GPU.cu:36-102, potential_GPU.cu:43-96,
+ * copy_CPU_btw_GPU.cu:35-138, links_CPU.cpp:33-83,
+ * allocate_cells_CPU.cpp:33-125, map_cells.cpp:128-159).
+ * With readconf the reference takes f and rho from the input (initial.cpp:78-83);
+ * compute_initial_forces = 1 evaluates them instead (the non-readconf path). */
+int sld_create(const sld_params* params, const sld_atoms* init,
+               int compute_initial_forces, sld_ctx** out);
+/* Replaces free_memory (free_memory.cpp:31-52). */
+int sld_destroy(sld_ctx* ctx);
+const char* sld_last_error(const sld_ctx* ctx);
+
+/* links (links_CPU.cpp:85-177 / links_GPU.cu:67-285). */
+int sld_links(sld_ctx* ctx);
+/* calculate_rho (calculate_rho_CPU.cpp:71-97). */
+int sld_calculate_rho(sld_ctx* ctx);
+/* calculate_force_energy (calculate_force_energy_CPU.cpp:220-301). */
+int sld_calculate_force_energy(sld_ctx* ctx);
+/* core_ds (core_ds_CPU.cpp:35-83). */
+int sld_core_ds(sld_ctx* ctx, double dt);
+/* core_dp (core_dp_CPU.cpp:51-85). */
+int sld_core_dp(sld_ctx* ctx, double dt);
+/* core_dr (core_dr_CPU.cpp:33-44). */
+int sld_core_dr(sld_ctx* ctx, double dt);
+/* core_dTe (core_dTe_CPU.cpp:47-64); needs eltemp. */
+int sld_core_dTe(sld_ctx* ctx, double dt);
+/* calculate_temperature (calculate_temperature_CPU.cpp:35-148); needs eltemp. */
+int sld_calculate_temperature(sld_ctx* ctx);
+/* check (check.cpp:31-47): check_energy (with the renormalizeEnergy feedback,
+ * check_energy_CPU.cpp:77-111), check_temperature, check_spin. step = -1
+ * records the initial energy. Writes the observables to *out (may be NULL). */
+int sld_check(sld_ctx* ctx, int step, sld_obs* out);
+/* nsteps of core() (core.cpp:31-76), each followed by total_time += step and
+ * check() on printed steps, as the driver loop does (spilady.cpp:50-57).
+ * obs (may be NULL) receives one entry per printed step, in order; *nobs
+ * (may be NULL) the number written. */
+int sld_step(sld_ctx* ctx, int first_step, int nsteps, sld_obs* obs, int* nobs);
+
+/* Device <-> host state (copy_atoms_from_GPU_to_CPU, copy_CPU_btw_GPU.cu:141-150).
+ * Atoms are in the caller's original index order. */
+int sld_download(sld_ctx* ctx, sld_atoms* out);
+/* Overwrites r, p, s (and f, rho when non-NULL) and rebuilds nothing else:
+ * the cell lists keep their current order. */
+int sld_upload(sld_ctx* ctx, const sld_atoms* in);
+int sld_get_cells(sld_ctx* ctx, sld_cells* out);
+int sld_set_cells_te(sld_ctx* ctx, const double* te);
+/* Link-cell geometry and groups (map_cells.cpp:41-60, allocate_cells_CPU.cpp). */
+int sld_get_geometry(sld_ctx* ctx, int* ncell3, int* ngroups, int* group_sizes);
+/* Per-cell linked lists, head to tail, as atom indices; cap entries per cell. */
+int sld_get_cell_lists(sld_ctx* ctx, int cap, int* counts, int* lists);
+double sld_total_time(const sld_ctx* ctx);
+
+/* Timing of the device work issued by the last sld_step call, and the
+ * per-kernel average duration (ms) of the named kernel family over that
+ * call, measured with HIP events on the context's stream. */
+int sld_last_step_timing(sld_ctx* ctx, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPILADY_HIP_H */
