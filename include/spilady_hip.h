@@ -152,10 +152,15 @@ int sld_get_geometry(sld_ctx* ctx, int* ncell3, int* ngroups, int* group_sizes);
 int sld_get_cell_lists(sld_ctx* ctx, int cap, int* counts, int* lists);
 double sld_total_time(const sld_ctx* ctx);
 
-/* Timing of the device work issued by the last sld_step call, and the
- * per-kernel average duration (ms) of the named kernel family over that
- * call, measured with HIP events on the context's stream. */
+/* Device time of the work issued by the last sld_step call, measured with
+ * HIP events on the context's stream. */
 int sld_last_step_timing(sld_ctx* ctx, double* total_ms);
+/* Bracket every launch of the kernel named `kernel` (e.g. "k_force"; "" or
+ * NULL turns it off) with HIP events on the context's stream; after each
+ * sld_step, sld_kernel_timing returns the launches seen and their summed
+ * device time over that call. */
+int sld_time_kernel(sld_ctx* ctx, const char* kernel);
+int sld_kernel_timing(sld_ctx* ctx, int* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,9 @@ class AtomBuffers:
 
     @classmethod
     def from_state(cls, st) -> "AtomBuffers":
+        """From an io.AtomState or a dict of arrays."""
+        if isinstance(st, dict):
+            return cls.from_dict(st, len(st["r"]))
         d = {k: getattr(st, k, None) for k in VEC_FIELDS + ("rho", "pe", "ke", "me", "me0", "stress")}
         return cls.from_dict(d, st.natom)
 

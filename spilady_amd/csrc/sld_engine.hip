@@ -1,0 +1,903 @@
+// Context, set-up and the C ABI of libspilady_hip.so (include/spilady_hip.h).
+//
+// The host side replaces the reference's global state (global.h:53-214) with
+// one context per run, keeps the reference's call order (core.cpp:31-76,
+// spilady.cpp:43-57) and issues every device step on the context's stream
+// without host synchronisation; capacity and finiteness flags are read back
+// once per sld_step call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spilady_hip.h"
+#include "sld_kernels.h"
+#include "sld_tables.h"
+
+using namespace sld;
+
+namespace {
+
+struct LevelSet {
+    std::vector<int> cells;   // cells grouped by level
+    std::vector<int> off;     // level offsets (size nlev + 1)
+    int* d_cells = nullptr;
+    int nlev() const { return (int)off.size() - 1; }
+};
+
+}  // namespace
+
+struct sld_ctx {
+    sld_params prm{};
+    Box box{};
+    Phys phys{};
+    Tables tab{};
+    Dev D{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int ngroups = 0;
+    std::vector<int> group_size;
+    std::vector<int> group_cells;  // concatenated in group order
+    LevelSet spin_fwd, spin_bwd, dte_fwd, dte_bwd;
+    // buffers
+    double *d_rtab = nullptr, *d_rhotab = nullptr;
+    int *d_cl2 = nullptr, *d_cn2 = nullptr;  // second cell-list buffer
+    double* d_part = nullptr;                // block partials
+    double* d_scal = nullptr;                // reduced scalars
+    sld_obs* d_obs = nullptr;                // observables ring
+    int obs_cap = 0;
+    double* d_einit = nullptr;               // initial_ave_energy and err
+    double total_time = 0.0;
+    uint64_t pass = 0;                       // noise pass counter
+    bool nb_dirty = true;
+    double vol = 0.0, dx2 = 0, dy2 = 0, dz2 = 0;
+    double last_ms = 0.0;
+    // per-kernel event timing of one named kernel
+    std::string timed_name;
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;
+    int timed_launches = 0;
+    double timed_ms = 0.0;
+    std::string err;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int set_err(sld_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_err((ctx), SLD_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(sld_ctx* c, T** p, size_t n) {
+    void* q = nullptr;
+    HIPCHK(c, hipMalloc(&q, n * sizeof(T) + 64));
+    HIPCHK(c, hipMemsetAsync(q, 0, n * sizeof(T) + 64, c->stream));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return SLD_OK;
+}
+
+inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
+
+// Event pair around a launch of the timed kernel (see sld_time_kernel).
+hipEvent_t* timing_slot(sld_ctx* c, const char* name) {
+    if (c->timed_name.empty() || c->timed_name != name) return nullptr;
+    if (c->ev_used + 2 > (int)c->ev_pool.size()) {
+        if (c->ev_pool.size() >= 8192) return nullptr;
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            c->ev_pool.push_back(e);
+        }
+    }
+    hipEvent_t* ev = &c->ev_pool[c->ev_used];
+    c->ev_used += 2;
+    return ev;
+}
+
+#define LAUNCH(ctx, kern, grid, block, ...)                                              \
+    do {                                                                                 \
+        if ((grid) > 0) {                                                                \
+            hipEvent_t* ev_ = timing_slot((ctx), #kern);                                 \
+            if (ev_) (void)hipEventRecord(ev_[0], (ctx)->stream);                        \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__); \
+            hipError_t e_ = hipGetLastError();                                           \
+            if (e_ != hipSuccess)                                                        \
+                return set_err((ctx), SLD_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(e_)); \
+            if (ev_) (void)hipEventRecord(ev_[1], (ctx)->stream);                        \
+        }                                                                                \
+    } while (0)
+
+#define TRY(x)                     \
+    do {                           \
+        int rc_ = (x);             \
+        if (rc_ != SLD_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host-side set-up: cells, lists, groups, levels
+
+struct HostCells {
+    int nx, ny, nz, C;
+};
+
+// map_cells.cpp:41-60
+HostCells cell_counts(const double* d, double mlc) {
+    double a1 = std::sin(std::atan2(d[2], d[1]));
+    double a2 = std::sin(std::atan2(d[5], d[3]));
+    double a3 = std::sin(std::atan2(d[5], d[4]));
+    double ex = std::max(mlc / a1, mlc / a2);
+    double ey = std::max(mlc, mlc / a3);
+    double ez = mlc;
+    HostCells h;
+    h.nx = (int)(d[0] / ex);
+    h.ny = (int)(d[2] / ey);
+    h.nz = (int)(d[5] / ez);
+    h.C = h.nx * h.ny * h.nz;
+    return h;
+}
+
+void host_neigh(const HostCells& h, int c, int out[26]) {
+    static const int off[26][3] = {
+        {1, 0, 0},  {1, 1, 0},  {0, 1, 0},  {-1, 1, 0},  {1, 0, -1}, {1, 1, -1}, {0, 1, -1},
+        {-1, 1, -1}, {1, 0, 1}, {1, 1, 1},  {0, 1, 1},   {-1, 1, 1}, {0, 0, 1},  {-1, 0, 0},
+        {0, -1, 0}, {0, 0, -1}, {1, -1, 0}, {-1, -1, 0}, {-1, 0, 1}, {-1, 0, -1}, {0, -1, 1},
+        {0, -1, -1}, {1, -1, 1}, {1, -1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+    int ix = c % h.nx, iy = (c / h.nx) % h.ny, iz = c / (h.nx * h.ny);
+    for (int s = 0; s < 26; ++s) {
+        int x = (ix + off[s][0] + h.nx) % h.nx, y = (iy + off[s][1] + h.ny) % h.ny,
+            z = (iz + off[s][2] + h.nz) % h.nz;
+        out[s] = x + y * h.nx + z * h.nx * h.ny;
+    }
+}
+
+// allocate_cells_CPU.cpp:33-125: greedy colouring in cell index order
+void greedy_groups(const HostCells& h, std::vector<int>& gsize, std::vector<int>& gcells,
+                   std::vector<int>& group_of) {
+    std::vector<int> type(h.C, 0);
+    group_of.assign(h.C, -1);
+    gsize.clear();
+    gcells.clear();
+    int left = h.C;
+    int nb[26];
+    while (left > 0) {
+        int nm = 0;
+        int g = (int)gsize.size();
+        for (int i = 0; i < h.C; ++i) {
+            if (type[i] != 0) continue;
+            type[i] = 1;
+            gcells.push_back(i);
+            group_of[i] = g;
+            ++nm;
+            --left;
+            host_neigh(h, i, nb);
+            for (int j = 0; j < 26; ++j)
+                if (type[nb[j]] != 1) type[nb[j]] = -1;
+        }
+        for (int i = 0; i < h.C; ++i)
+            if (type[i] == -1) type[i] = 0;
+        gsize.push_back(nm);
+    }
+}
+
+// Dependency levels of a group-ordered Gauss-Seidel sweep. A cell reads the
+// cells in `slots`; its update must follow those of earlier groups and
+// precede those of later groups. level(c) = 1 + max level of its earlier-group
+// readers; cells of one level never read each other, so one launch per level
+// reproduces the sweep exactly.
+void make_levels(const HostCells& h, const std::vector<int>& gsize, const std::vector<int>& gcells,
+                 const std::vector<int>& group_of, const int* slots, int nslots, bool backward,
+                 LevelSet& L) {
+    std::vector<int> level(h.C, -1);
+    int G = (int)gsize.size();
+    std::vector<int> goff(G + 1, 0);
+    for (int g = 0; g < G; ++g) goff[g + 1] = goff[g] + gsize[g];
+    int maxl = -1;
+    int nb[26];
+    for (int gg = 0; gg < G; ++gg) {
+        int g = backward ? G - 1 - gg : gg;
+        for (int k = goff[g]; k < goff[g + 1]; ++k) {
+            int c = gcells[k];
+            host_neigh(h, c, nb);
+            int lv = 0;
+            for (int s = 0; s < nslots; ++s) {
+                int n = nb[slots[s]];
+                int gn = group_of[n];
+                bool earlier = backward ? gn > g : gn < g;
+                if (earlier && level[n] + 1 > lv) lv = level[n] + 1;
+            }
+            level[c] = lv;
+            maxl = std::max(maxl, lv);
+        }
+    }
+    L.off.assign(maxl + 2, 0);
+    for (int c = 0; c < h.C; ++c) L.off[level[c] + 1]++;
+    for (int l = 0; l <= maxl; ++l) L.off[l + 1] += L.off[l];
+    L.cells.assign(h.C, 0);
+    std::vector<int> fill(L.off.begin(), L.off.end() - 1);
+    // keep group order inside a level
+    for (int gg = 0; gg < G; ++gg) {
+        int g = backward ? G - 1 - gg : gg;
+        for (int k = goff[g]; k < goff[g + 1]; ++k) {
+            int c = gcells[k];
+            L.cells[fill[level[c]]++] = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device building blocks
+
+int reduce(sld_ctx* c, int nblocks, int nk, double* out) {
+    LAUNCH(c, k_reduce, 1, 256, c->d_part, nblocks, nk, out);
+    return SLD_OK;
+}
+
+int nbuild(sld_ctx* c, int write_rho) {
+    LAUNCH(c, k_nbuild, nblk(c->D.N, 256), 256, c->box, c->tab, c->phys, c->D, write_rho);
+    c->nb_dirty = false;
+    return SLD_OK;
+}
+
+int ensure_nb(sld_ctx* c) {
+    if (c->nb_dirty) return nbuild(c, 0);
+    return SLD_OK;
+}
+
+int do_links(sld_ctx* c) {
+    LAUNCH(c, k_bin, nblk(c->D.N, 256), 256, c->box, c->D);
+    // swap list buffers: the old lists become the input of the rebuild
+    std::swap(c->D.cl, c->d_cl2);
+    std::swap(c->D.cn, c->d_cn2);
+    LAUNCH(c, k_relink, nblk(c->D.C, 128), 128, c->box, c->d_cl2, c->d_cn2, c->D);
+    return SLD_OK;
+}
+
+int do_force(sld_ctx* c) {
+    TRY(ensure_nb(c));
+    LAUNCH(c, k_fp, nblk(c->D.N, 256), 256, c->tab, c->D);
+    LAUNCH(c, k_force, nblk(c->D.N, 256), 256, c->box, c->tab, c->phys, c->D);
+    return SLD_OK;
+}
+
+int do_ds(sld_ctx* c, double dt) {
+    TRY(ensure_nb(c));
+    double dth = dt / 2.0;
+    for (int dir = 0; dir < 2; ++dir) {
+        LevelSet& L = dir == 0 ? c->spin_fwd : c->spin_bwd;
+        for (int l = 0; l < L.nlev(); ++l) {
+            int n = L.off[l + 1] - L.off[l];
+            LAUNCH(c, k_spin_phase, nblk(n, 64), 64, L.d_cells + L.off[l], n, dir, dth, c->tab,
+                   c->phys, c->D, c->pass);
+        }
+        c->pass++;
+    }
+    return SLD_OK;
+}
+
+int do_dp(sld_ctx* c, double dt) {
+    int nb = nblk(c->D.C, 256);
+    LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, c->pass, c->d_part);
+    if (c->phys.lattlang) c->pass++;
+    TRY(reduce(c, nb, 3, c->d_scal));
+    int na = nblk(c->D.N, 256);
+    LAUNCH(c, k_dp_center, na, 256, c->phys, c->D, (const double*)c->d_scal, c->d_part);
+    TRY(reduce(c, na, 2, c->d_scal + 3));
+    LAUNCH(c, k_dp_scale, na, 256, c->phys, c->D, (const double*)(c->d_scal + 3));
+    return SLD_OK;
+}
+
+int do_dr(sld_ctx* c, double dt) {
+    LAUNCH(c, k_dr, nblk(c->D.N, 256), 256, dt, c->box, c->phys, c->D);
+    c->nb_dirty = true;
+    return SLD_OK;
+}
+
+int do_temperature(sld_ctx* c) {
+    TRY(ensure_nb(c));
+    LAUNCH(c, k_temperature, nblk(c->D.C, 128), 128, c->tab, c->phys, c->D);
+    return SLD_OK;
+}
+
+// core_dTe_CPU.cpp:47-217 with nloop = 50
+int do_dTe(sld_ctx* c, double dt) {
+    const int nloop = 50;
+    double dtl = dt / nloop;
+    double h = dtl / 2.0;
+    double kappa = c->prm.kappa_e / kEvToJ / 1e10;
+    double atomic_volume = c->vol / (double)c->D.N;
+    double vf = atomic_volume / ((double)c->D.N / (double)c->D.C);
+    double pref = dtl / 2.0 * kappa * vf / kBoltz;
+    int nb = nblk(c->D.C, 256);
+    for (int it = 0; it < nloop; ++it) {
+        LAUNCH(c, k_dte_ab, nb, 256, h, c->phys, c->D, c->d_part);
+        TRY(reduce(c, nb, 1, c->d_scal + 8));
+        for (int dir = 0; dir < 2; ++dir) {
+            LevelSet& L = dir == 0 ? c->dte_fwd : c->dte_bwd;
+            for (int l = 0; l < L.nlev(); ++l) {
+                int n = L.off[l + 1] - L.off[l];
+                LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, c->box, pref,
+                       c->dx2, c->dy2, c->dz2, c->D);
+            }
+        }
+        LAUNCH(c, k_dte_end, nb, 256, c->D, c->d_part);
+        TRY(reduce(c, nb, 1, c->d_scal + 9));
+        LAUNCH(c, k_dte_finish, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8));
+    }
+    return SLD_OK;
+}
+
+}  // namespace
+
+// observables record from the reduced sums; renormalizeEnergy error
+// (check_energy_CPU.cpp:33-111, check_temperature_CPU.cpp:33-134,
+// check_spin_CPU.cpp:33-56)
+__global__ void k_obs_final(const double* sa, const double* sc, int N, int C, int eltemp, int renorm,
+                            int step, double total_time, double* einit, sld_obs* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sld_obs o = {};
+    o.step = step;
+    o.total_time = total_time;
+    o.ave_pe = sa[0] / N;
+    o.ave_ke = sa[1] / N;
+    o.ave_me = sa[2] / N;
+    o.ave_me0 = sa[3] / N;
+    double err = 0.0;
+    if (eltemp) {
+        double apc = (double)N / (double)C;
+        o.ave_ee = apc * sc[0] / N;
+        if (renorm) {
+            double e = 0.0;
+            e += o.ave_pe; e += o.ave_ke; e += o.ave_me; e += o.ave_me0; e += o.ave_ee;
+            if (step == -1) einit[0] = e;
+            else err = e - einit[0];
+        }
+    }
+    einit[1] = err;
+    o.tl = sa[1] * 2.0 / 3.0 / N / kBoltz;
+    double rup = eltemp ? sc[1] : sa[7], rdn = eltemp ? sc[2] : sa[8];
+    double lup = eltemp ? sc[3] : sa[9], ldn = eltemp ? sc[4] : sa[10];
+    o.ts_r = rup / rdn / 2.0 / kBoltz;
+    o.ts_l = lup / ldn / kBoltz;
+    if (eltemp) o.te = sc[5] / C / kBoltz;
+    double sx = sa[4] / N, sy = sa[5] / N, sz = sa[6] / N;
+    o.ave_s[0] = sx; o.ave_s[1] = sy; o.ave_s[2] = sz;
+    o.ave_s0 = sqrt(sx * sx + sy * sy + sz * sz);
+    o.ave_m[0] = -kElG * sx; o.ave_m[1] = -kElG * sy; o.ave_m[2] = -kElG * sz;
+    o.ave_m0 = sqrt(o.ave_m[0] * o.ave_m[0] + o.ave_m[1] * o.ave_m[1] + o.ave_m[2] * o.ave_m[2]);
+    o.virial = sa[11];
+    *out = o;
+}
+
+__global__ void k_obs_set_te(const double* te_sum, int C, sld_obs* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) out->te = te_sum[0] / C / kBoltz;
+}
+
+namespace {
+
+int do_check(sld_ctx* c, int step, sld_obs* d_out) {
+    int na = nblk(c->D.N, 256);
+    int scan = c->phys.eltemp ? 0 : 1;
+    if (scan) TRY(ensure_nb(c));
+    LAUNCH(c, k_obs_atoms, na, 256, c->tab, c->phys, c->D, c->d_part, scan);
+    TRY(reduce(c, na, kObsAtomK + 1, c->d_scal + 16));
+    if (c->phys.eltemp) {
+        int nb = nblk(c->D.C, 256);
+        LAUNCH(c, k_obs_cells, nb, 256, c->D, c->d_part);
+        TRY(reduce(c, nb, kObsCellK, c->d_scal + 32));
+    }
+    LAUNCH(c, k_obs_final, 1, 64, (const double*)(c->d_scal + 16), (const double*)(c->d_scal + 32),
+           c->D.N, c->D.C, c->phys.eltemp, c->prm.renormalize_energy, step, c->total_time,
+           c->d_einit, d_out);
+    if (c->phys.eltemp && c->prm.renormalize_energy) {
+        int nb = nblk(c->D.C, 256);
+        LAUNCH(c, k_renorm, nb, 256, c->D, (const double*)(c->d_einit + 1), c->d_part);
+        TRY(reduce(c, nb, 1, c->d_scal + 40));
+        LAUNCH(c, k_obs_set_te, 1, 64, (const double*)(c->d_scal + 40), c->D.C, d_out);
+    }
+    return SLD_OK;
+}
+
+// core.cpp:31-76 (SLDHL)
+int do_core(sld_ctx* c) {
+    double dt = c->prm.step;
+    TRY(do_links(c));
+    if (c->phys.eltemp) TRY(do_dTe(c, dt / 2.0));
+    TRY(do_dp(c, dt / 2.0));
+    TRY(do_ds(c, dt / 2.0));
+    TRY(do_dr(c, dt));
+    TRY(nbuild(c, 1));  // calculate_rho
+    TRY(do_ds(c, dt / 2.0));
+    TRY(do_force(c));
+    TRY(do_dp(c, dt / 2.0));
+    if (c->phys.eltemp) {
+        TRY(do_temperature(c));
+        TRY(do_dTe(c, dt / 2.0));
+    }
+    return SLD_OK;
+}
+
+int read_flags(sld_ctx* c) {
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->D.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flags & kErrCellOverflow)
+        return set_err(c, SLD_ERR_CAPACITY, "a link cell exceeded %d atoms", kCellCap);
+    if (flags & kErrNbOverflow)
+        return set_err(c, SLD_ERR_CAPACITY, "an atom exceeded %d neighbours within rcut_max or %d within rcut_mag",
+                       kMaxNb, kMaxMag);
+    if (flags & kErrFarMove)
+        return set_err(c, SLD_ERR_STATE, "an atom moved beyond the neighbouring link cells in one step");
+    if (flags & kErrNonFinite)
+        return set_err(c, SLD_ERR_STATE, "non-finite position, momentum or spin");
+    return SLD_OK;
+}
+
+void pack4(const double* src, std::vector<double>& dst, int n, bool with_len) {
+    dst.assign((size_t)n * 4, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
+        dst[4 * i] = x; dst[4 * i + 1] = y; dst[4 * i + 2] = z;
+        dst[4 * i + 3] = with_len ? std::sqrt(x * x + y * y + z * z) : 0.0;
+    }
+}
+
+int up4(sld_ctx* c, double4* d, const double* src, bool with_len) {
+    if (!src) return SLD_OK;
+    std::vector<double> h;
+    pack4(src, h, c->D.N, with_len);
+    HIPCHK(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
+int up1(sld_ctx* c, double* d, const double* src, size_t n) {
+    if (!src) return SLD_OK;
+    HIPCHK(c, hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return SLD_OK;
+}
+
+int down4(sld_ctx* c, const double4* d, double* dst) {
+    if (!dst) return SLD_OK;
+    std::vector<double> h((size_t)c->D.N * 4);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->D.N; ++i) {
+        dst[3 * i] = h[4 * i]; dst[3 * i + 1] = h[4 * i + 1]; dst[3 * i + 2] = h[4 * i + 2];
+    }
+    return SLD_OK;
+}
+
+int down1(sld_ctx* c, const double* d, double* dst, size_t n) {
+    if (!dst) return SLD_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
+int upload_levels(sld_ctx* c, LevelSet& L) {
+    TRY(dalloc(c, &L.d_cells, L.cells.size()));
+    HIPCHK(c, hipMemcpyAsync(L.d_cells, L.cells.data(), L.cells.size() * sizeof(int),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
+int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int compute_forces) {
+    c->prm = *p;
+    sld_params& P = c->prm;
+    if (P.natom <= 0) return set_err(c, SLD_ERR_ARG, "natom must be positive");
+    if (!in || !in->r || !in->p || !in->s) return set_err(c, SLD_ERR_ARG, "r, p and s are required");
+    if (!(P.box[0] > 0 && P.box[2] > 0 && P.box[5] > 0)) return set_err(c, SLD_ERR_ARG, "box diagonal must be positive");
+    if (P.rcut_max < P.rcut_pot || P.rcut_max < P.rcut_mag)
+        return set_err(c, SLD_ERR_ARG, "rcut_max must be the largest cutoff");
+    if (P.rcut_max > P.min_length_link_cell) P.min_length_link_cell = P.rcut_max + 0.001;
+    if (P.localcolmot && !P.eltemp) P.localcolmot = 0;
+    if (P.renormalize_energy && !P.eltemp) P.renormalize_energy = 0;
+    if (P.step <= 0) return set_err(c, SLD_ERR_ARG, "step must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(c, SLD_ERR_NOGPU, "no HIP device");
+    HIPCHK(c, hipGetDevice(&c->device));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // geometry (read_config.cpp:110-116, vec_utils_CPU.cpp:135-148, map_cells.cpp:41-60)
+    const double* d = P.box;
+    Box& b = c->box;
+    for (int k = 0; k < 6; ++k) b.d[k] = d[k];
+    b.inv[0] = 1e0 / d[0];
+    b.inv[1] = -d[1] / (d[0] * d[2]);
+    b.inv[2] = 1e0 / d[2];
+    b.inv[3] = (d[1] * d[4] - d[2] * d[3]) / (d[0] * d[2] * d[5]);
+    b.inv[4] = -d[4] / (d[2] * d[5]);
+    b.inv[5] = 1e0 / d[5];
+    HostCells hc = cell_counts(d, P.min_length_link_cell);
+    if (hc.nx < 3 || hc.ny < 3 || hc.nz < 3)
+        return set_err(c, SLD_ERR_ARG, "need at least 3 link cells per side (got %d %d %d)", hc.nx, hc.ny, hc.nz);
+    b.nx = hc.nx; b.ny = hc.ny; b.nz = hc.nz;
+    const int N = P.natom, C = hc.C;
+    c->vol = d[0] * d[2] * d[5];
+    double blx = std::fabs(d[0]);
+    double bly = std::sqrt(d[1] * d[1] + d[2] * d[2]);
+    double blz = std::sqrt(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+    c->dx2 = std::pow(blx / hc.nx, 2);
+    c->dy2 = std::pow(bly / hc.ny, 2);
+    c->dz2 = std::pow(blz / hc.nz, 2);
+
+    Phys& ph = c->phys;
+    ph.atmass = P.atmass;
+    ph.gamma_L = P.gamma_L_over_mass * P.atmass;
+    ph.gamma_S = P.gamma_S_HL;
+    ph.temperature = P.temperature;
+    ph.rcut_pot = P.rcut_pot;
+    ph.rcut_mag = P.rcut_mag;
+    ph.rcut_pot_sq = std::pow(P.rcut_pot, 2);
+    ph.rcut_mag_sq = std::pow(P.rcut_mag, 2);
+    ph.rcut_max_sq = std::pow(P.rcut_max, 2);
+    ph.box_volume = c->vol;
+    ph.seed = P.random_seed;
+    ph.replica = P.replica;
+    ph.spinlang = P.spinlang;
+    ph.lattlang = P.lattlang;
+    ph.eltemp = P.eltemp;
+    ph.localcolmot = P.localcolmot;
+
+    // tables
+    HostTables ht;
+    build_tables(P.rcut_max, ht);
+    TRY(dalloc(c, &c->d_rtab, ht.rtab.size()));
+    TRY(dalloc(c, &c->d_rhotab, ht.rhotab.size()));
+    HIPCHK(c, hipMemcpyAsync(c->d_rtab, ht.rtab.data(), ht.rtab.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_rhotab, ht.rhotab.data(), ht.rhotab.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->tab.rtab = c->d_rtab;
+    c->tab.rhotab = c->d_rhotab;
+    c->tab.in_over_rmax = ht.in_over_rmax;
+    c->tab.in_over_rhomax = ht.in_over_rhomax;
+
+    // device arrays
+    Dev& D = c->D;
+    D.N = N;
+    D.C = C;
+    TRY(dalloc(c, &D.r, N)); TRY(dalloc(c, &D.p, N)); TRY(dalloc(c, &D.f, N));
+    TRY(dalloc(c, &D.s, N)); TRY(dalloc(c, &D.hh, N)); TRY(dalloc(c, &D.hl, N));
+    TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
+    TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
+    TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
+    TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N));
+    TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
+    TRY(dalloc(c, &c->d_cl2, (size_t)C * kCellCap)); TRY(dalloc(c, &c->d_cn2, C));
+    TRY(dalloc(c, &D.nb, (size_t)N * kMaxNb)); TRY(dalloc(c, &D.nbn, N));
+    TRY(dalloc(c, &D.nbm, (size_t)N * kMaxMag)); TRY(dalloc(c, &D.nbmn, N));
+    TRY(dalloc(c, &D.nbJ, (size_t)N * kMaxMag));
+    TRY(dalloc(c, &D.Te, C)); TRY(dalloc(c, &D.Ee, C)); TRY(dalloc(c, &D.Tl, C));
+    TRY(dalloc(c, &D.Gel, C)); TRY(dalloc(c, &D.Ges, C)); TRY(dalloc(c, &D.TsR, C));
+    TRY(dalloc(c, &D.TsL, C)); TRY(dalloc(c, &D.sRup, C)); TRY(dalloc(c, &D.sRdn, C));
+    TRY(dalloc(c, &D.sLup, C)); TRY(dalloc(c, &D.sLdn, C)); TRY(dalloc(c, &D.cellv, C));
+    TRY(dalloc(c, &D.err, 1));
+    int maxblk = std::max(nblk(N, 256), nblk(C, 128)) + 1;
+    TRY(dalloc(c, &c->d_part, (size_t)maxblk * 16));
+    TRY(dalloc(c, &c->d_scal, 64));
+    TRY(dalloc(c, &c->d_einit, 4));
+    c->obs_cap = 1;
+    TRY(dalloc(c, &c->d_obs, 1));
+
+    // atoms: positions wrapped as initial_links does (links_CPU.cpp:37-50)
+    std::vector<double> r3(in->r, in->r + 3 * (size_t)N);
+    std::vector<int> cell(N);
+    for (int i = 0; i < N; ++i) {
+        double x = r3[3 * i], y = r3[3 * i + 1], z = r3[3 * i + 2];
+        double q[3] = {b.inv[0] * x + b.inv[1] * y + b.inv[3] * z, b.inv[2] * y + b.inv[4] * z, b.inv[5] * z};
+        int idx = 0;
+        for (int k = 0; k < 3; ++k)
+            if (q[k] < 0e0 || q[k] >= 1e0) { q[k] -= (int)q[k]; ++idx; }
+        if (idx > 0) {
+            r3[3 * i] = d[0] * q[0] + d[1] * q[1] + d[3] * q[2];
+            r3[3 * i + 1] = d[2] * q[1] + d[4] * q[2];
+            r3[3 * i + 2] = d[5] * q[2];
+        }
+        x = r3[3 * i]; y = r3[3 * i + 1]; z = r3[3 * i + 2];
+        double qx = b.inv[0] * x + b.inv[1] * y + b.inv[3] * z, qy = b.inv[2] * y + b.inv[4] * z, qz = b.inv[5] * z;
+        int lx = (int)(qx * hc.nx), ly = (int)(qy * hc.ny), lz = (int)(qz * hc.nz);
+        lx = std::min(std::max(lx, 0), hc.nx - 1);
+        ly = std::min(std::max(ly, 0), hc.ny - 1);
+        lz = std::min(std::max(lz, 0), hc.nz - 1);
+        cell[i] = lx + ly * hc.nx + lz * hc.nx * hc.ny;
+    }
+    TRY(up4(c, D.r, r3.data(), false));
+    TRY(up4(c, D.p, in->p, false));
+    TRY(up4(c, D.f, in->f, false));
+    TRY(up4(c, D.s, in->s, true));
+    TRY(up4(c, D.hh, in->heff_h, false));
+    TRY(up4(c, D.hl, in->heff_l, false));
+    TRY(up1(c, D.rho, in->rho, N));
+    TRY(up1(c, D.pe, in->pe, N));
+    TRY(up1(c, D.ke, in->ke, N));
+    TRY(up1(c, D.me, in->me, N));
+    TRY(up1(c, D.me0, in->me0, N));
+    if (in->stress) {
+        std::vector<double> st((size_t)6 * N);
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < 6; ++k) st[(size_t)k * N + i] = in->stress[6 * i + k];
+        HIPCHK(c, hipMemcpyAsync(D.stress, st.data(), st.size() * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+
+    // initial lists: each atom pushed at the head in ascending index, so a
+    // list holds its atoms in descending index (links_CPU.cpp:53-81)
+    std::vector<int> cn(C, 0), cl((size_t)C * kCellCap, 0);
+    for (int i = N - 1; i >= 0; --i) {
+        int cc = cell[i];
+        if (cn[cc] >= kCellCap) return set_err(c, SLD_ERR_CAPACITY, "a link cell exceeds %d atoms at start", kCellCap);
+        cl[(size_t)cc * kCellCap + cn[cc]++] = i;
+    }
+    HIPCHK(c, hipMemcpyAsync(D.cell, cell.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.cell_prev, cell.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.cn, cn.data(), C * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.cl, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+
+    // groups and sweep levels
+    std::vector<int> group_of;
+    greedy_groups(hc, c->group_size, c->group_cells, group_of);
+    c->ngroups = (int)c->group_size.size();
+    int all26[26];
+    for (int k = 0; k < 26; ++k) all26[k] = k;
+    const int faces[6] = {0, 13, 2, 14, 12, 15};
+    make_levels(hc, c->group_size, c->group_cells, group_of, all26, 26, false, c->spin_fwd);
+    make_levels(hc, c->group_size, c->group_cells, group_of, all26, 26, true, c->spin_bwd);
+    make_levels(hc, c->group_size, c->group_cells, group_of, faces, 6, false, c->dte_fwd);
+    make_levels(hc, c->group_size, c->group_cells, group_of, faces, 6, true, c->dte_bwd);
+    TRY(upload_levels(c, c->spin_fwd));
+    TRY(upload_levels(c, c->spin_bwd));
+    TRY(upload_levels(c, c->dte_fwd));
+    TRY(upload_levels(c, c->dte_bwd));
+
+    // electron temperatures (map_cells.cpp:128-159)
+    if (P.eltemp) {
+        std::vector<double> te(C, P.temperature), ee(C);
+        const double bb = 2e-4 / kBoltz;
+        for (int k = 0; k < C; ++k) ee[k] = 3e0 / bb * std::log(std::cosh(bb * te[k]));
+        TRY(up1(c, D.Te, te.data(), C));
+        TRY(up1(c, D.Ee, ee.data(), C));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    // initial.cpp:78-87
+    TRY(nbuild(c, compute_forces ? 1 : 0));
+    if (compute_forces) TRY(do_force(c));
+    if (P.eltemp) TRY(do_temperature(c));
+    c->total_time = P.start_time;
+    c->pass = 0;
+    TRY(do_check(c, -1, c->d_obs));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return read_flags(c);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int sld_version(void) { return SLD_ABI_VERSION; }
+
+int sld_device_info(char* name, int len, int* n_cu) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess) return SLD_ERR_NOGPU;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return SLD_ERR_NOGPU;
+    if (name && len > 0) {
+        snprintf(name, len, "%s", prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    return SLD_OK;
+}
+
+int sld_create(const sld_params* params, const sld_atoms* init, int compute_initial_forces, sld_ctx** out) {
+    if (!params || !out) return SLD_ERR_ARG;
+    *out = nullptr;
+    sld_ctx* c = new sld_ctx();
+    int rc = create_impl(c, params, init, compute_initial_forces);
+    *out = c;  // returned even on failure so that sld_last_error can be read
+    return rc;
+}
+
+int sld_destroy(sld_ctx* c) {
+    if (!c) return SLD_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SLD_OK;
+}
+
+const char* sld_last_error(const sld_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sld_links(sld_ctx* c) { TRY(do_links(c)); return read_flags(c); }
+int sld_calculate_rho(sld_ctx* c) { TRY(nbuild(c, 1)); return read_flags(c); }
+int sld_calculate_force_energy(sld_ctx* c) { TRY(do_force(c)); return read_flags(c); }
+int sld_core_ds(sld_ctx* c, double dt) { TRY(do_ds(c, dt)); return read_flags(c); }
+int sld_core_dp(sld_ctx* c, double dt) { TRY(do_dp(c, dt)); return read_flags(c); }
+int sld_core_dr(sld_ctx* c, double dt) { TRY(do_dr(c, dt)); return read_flags(c); }
+
+int sld_core_dTe(sld_ctx* c, double dt) {
+    if (!c->phys.eltemp) return set_err(c, SLD_ERR_STATE, "core_dTe needs eltemp");
+    TRY(do_dTe(c, dt));
+    return read_flags(c);
+}
+
+int sld_calculate_temperature(sld_ctx* c) {
+    if (!c->phys.eltemp) return set_err(c, SLD_ERR_STATE, "calculate_temperature needs eltemp");
+    TRY(do_temperature(c));
+    return read_flags(c);
+}
+
+int sld_check(sld_ctx* c, int step, sld_obs* out) {
+    TRY(do_check(c, step, c->d_obs));
+    if (out) HIPCHK(c, hipMemcpyAsync(out, c->d_obs, sizeof(sld_obs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return read_flags(c);
+}
+
+int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
+    if (nsteps < 0) return set_err(c, SLD_ERR_ARG, "nsteps < 0");
+    int every = c->prm.interval_of_print_out > 0 ? c->prm.interval_of_print_out : 1;
+    int nprint = 0;
+    for (int s = first_step; s < first_step + nsteps; ++s)
+        if ((s + 1) % every == 0) ++nprint;
+    if (nprint > c->obs_cap) {
+        sld_obs* nb = nullptr;
+        HIPCHK(c, hipMalloc(&nb, sizeof(sld_obs) * nprint));
+        c->allocs.push_back(nb);
+        c->d_obs = nb;
+        c->obs_cap = nprint;
+    }
+    c->ev_used = 0;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    int k = 0;
+    for (int s = first_step; s < first_step + nsteps; ++s) {
+        TRY(do_core(c));
+        c->total_time += c->prm.step;
+        if ((s + 1) % every == 0) {
+            TRY(do_check(c, s, c->d_obs + k));
+            ++k;
+        }
+    }
+    LAUNCH(c, k_check_nonfinite, nblk(c->D.N, 256), 256, c->D);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    c->last_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->timed_launches = c->ev_used / 2;
+    c->timed_ms = 0.0;
+    for (int k = 0; k + 1 < c->ev_used; k += 2) {
+        float kms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_pool[k], c->ev_pool[k + 1]));
+        c->timed_ms += kms;
+    }
+    if (obs && k > 0)
+        HIPCHK(c, hipMemcpy(obs, c->d_obs, sizeof(sld_obs) * k, hipMemcpyDeviceToHost));
+    if (nobs) *nobs = k;
+    return read_flags(c);
+}
+
+int sld_last_step_timing(sld_ctx* c, double* total_ms) {
+    if (total_ms) *total_ms = c->last_ms;
+    return SLD_OK;
+}
+
+int sld_time_kernel(sld_ctx* c, const char* kernel) {
+    c->timed_name = kernel ? kernel : "";
+    c->ev_used = 0;
+    return SLD_OK;
+}
+
+int sld_kernel_timing(sld_ctx* c, int* launches, double* total_ms) {
+    if (launches) *launches = c->timed_launches;
+    if (total_ms) *total_ms = c->timed_ms;
+    return SLD_OK;
+}
+
+int sld_download(sld_ctx* c, sld_atoms* o) {
+    if (!o) return SLD_ERR_ARG;
+    Dev& D = c->D;
+    const int N = D.N;
+    TRY(down4(c, D.r, o->r)); TRY(down4(c, D.p, o->p)); TRY(down4(c, D.f, o->f));
+    TRY(down4(c, D.s, o->s)); TRY(down4(c, D.hh, o->heff_h)); TRY(down4(c, D.hl, o->heff_l));
+    TRY(down1(c, D.rho, o->rho, N)); TRY(down1(c, D.pe, o->pe, N)); TRY(down1(c, D.ke, o->ke, N));
+    TRY(down1(c, D.me, o->me, N)); TRY(down1(c, D.me0, o->me0, N)); TRY(down1(c, D.vir, o->vir, N));
+    TRY(down1(c, D.sjs, o->sum_jij_sj, N));
+    if (o->stress) {
+        std::vector<double> st((size_t)6 * N);
+        TRY(down1(c, D.stress, st.data(), st.size()));
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < 6; ++k) o->stress[6 * i + k] = st[(size_t)k * N + i];
+    }
+    return SLD_OK;
+}
+
+int sld_upload(sld_ctx* c, const sld_atoms* in) {
+    if (!in) return SLD_ERR_ARG;
+    Dev& D = c->D;
+    TRY(up4(c, D.r, in->r, false));
+    TRY(up4(c, D.p, in->p, false));
+    TRY(up4(c, D.s, in->s, true));
+    TRY(up4(c, D.f, in->f, false));
+    TRY(up1(c, D.rho, in->rho, D.N));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (in->r) c->nb_dirty = true;
+    return SLD_OK;
+}
+
+int sld_get_cells(sld_ctx* c, sld_cells* o) {
+    if (!o) return SLD_ERR_ARG;
+    Dev& D = c->D;
+    const int C = D.C;
+    TRY(down1(c, D.Te, o->te, C)); TRY(down1(c, D.Ee, o->ee, C)); TRY(down1(c, D.Tl, o->tl, C));
+    TRY(down1(c, D.TsR, o->ts_r, C)); TRY(down1(c, D.TsL, o->ts_l, C));
+    TRY(down1(c, D.Gel, o->gel, C)); TRY(down1(c, D.Ges, o->ges, C));
+    TRY(down1(c, D.sRup, o->sum_r_up, C)); TRY(down1(c, D.sRdn, o->sum_r_dn, C));
+    TRY(down1(c, D.sLup, o->sum_l_up, C)); TRY(down1(c, D.sLdn, o->sum_l_dn, C));
+    return SLD_OK;
+}
+
+int sld_set_cells_te(sld_ctx* c, const double* te) {
+    TRY(up1(c, c->D.Te, te, c->D.C));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
+int sld_get_geometry(sld_ctx* c, int* ncell3, int* ngroups, int* group_sizes) {
+    if (ncell3) { ncell3[0] = c->box.nx; ncell3[1] = c->box.ny; ncell3[2] = c->box.nz; }
+    if (ngroups) *ngroups = c->ngroups;
+    if (group_sizes) std::copy(c->group_size.begin(), c->group_size.end(), group_sizes);
+    return SLD_OK;
+}
+
+int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
+    const int C = c->D.C;
+    std::vector<int> cn(C), cl((size_t)C * kCellCap);
+    HIPCHK(c, hipMemcpyAsync(cn.data(), c->D.cn, C * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cl.data(), c->D.cl, cl.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int mx = 0;
+    for (int k = 0; k < C; ++k) {
+        if (counts) counts[k] = cn[k];
+        mx = std::max(mx, cn[k]);
+        if (lists)
+            for (int q = 0; q < std::min(cn[k], cap); ++q) lists[(size_t)k * cap + q] = cl[(size_t)k * kCellCap + q];
+    }
+    if (mx > cap) return set_err(c, SLD_ERR_CAPACITY, "cell holds %d atoms > cap %d", mx, cap);
+    return SLD_OK;
+}
+
+double sld_total_time(const sld_ctx* c) { return c->total_time; }
+
+}  // extern "C"

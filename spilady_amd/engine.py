@@ -1,0 +1,202 @@
+"""Python binding of libspilady_hip.so (include/spilady_hip.h) over ctypes.
+
+The engine runs only on a gfx950 device through the HIP library: there is no
+CPU fallback. Importing this module does not touch the GPU; creating an
+Engine does, and raises if the library is missing or no device is usable.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspilady_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "spilady_hip.h")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+ERRORS = {1: "SLD_ERR_ARG", 2: "SLD_ERR_HIP", 3: "SLD_ERR_CAPACITY", 4: "SLD_ERR_STATE",
+          5: "SLD_ERR_NOGPU"}
+
+
+class SldError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def exported_symbols() -> List[str]:
+    """Every function declared in include/spilady_hip.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sld_[a-z_0-9]+)\s*\(", txt)))
+
+
+def load(path: Optional[str] = None):
+    """Load the HIP library (no device call)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built: run __graft_entry__.build() "
+                                "(make -C spilady_amd/csrc)")
+    L = C.CDLL(p)
+    P = C.POINTER(abi.SldParams)
+    A = C.POINTER(abi.SldAtoms)
+    V = C.c_void_p
+    L.sld_version.restype = C.c_int
+    L.sld_device_info.argtypes = [C.c_char_p, C.c_int, _ip]
+    L.sld_create.argtypes = [P, A, C.c_int, C.POINTER(V)]
+    L.sld_destroy.argtypes = [V]
+    L.sld_last_error.argtypes = [V]
+    L.sld_last_error.restype = C.c_char_p
+    for nm in ("sld_links", "sld_calculate_rho", "sld_calculate_force_energy",
+               "sld_calculate_temperature"):
+        getattr(L, nm).argtypes = [V]
+    for nm in ("sld_core_ds", "sld_core_dp", "sld_core_dr", "sld_core_dTe"):
+        getattr(L, nm).argtypes = [V, C.c_double]
+    L.sld_check.argtypes = [V, C.c_int, C.POINTER(abi.SldObs)]
+    L.sld_step.argtypes = [V, C.c_int, C.c_int, C.POINTER(abi.SldObs), _ip]
+    L.sld_download.argtypes = [V, A]
+    L.sld_upload.argtypes = [V, A]
+    L.sld_get_cells.argtypes = [V, C.POINTER(abi.SldCells)]
+    L.sld_set_cells_te.argtypes = [V, _dp]
+    L.sld_get_geometry.argtypes = [V, _ip, _ip, _ip]
+    L.sld_get_cell_lists.argtypes = [V, C.c_int, _ip, _ip]
+    L.sld_total_time.argtypes = [V]
+    L.sld_total_time.restype = C.c_double
+    L.sld_last_step_timing.argtypes = [V, _dp]
+    L.sld_time_kernel.argtypes = [V, C.c_char_p]
+    L.sld_kernel_timing.argtypes = [V, _ip, _dp]
+    if path is None:
+        _lib = L
+    return L
+
+
+class Engine(abi.EngineLike):
+    """One spin-lattice run on the GPU (the reference's global state, as a
+    context). Method names follow the reference entry points."""
+
+    def __init__(self, params: abi.SldParams, atoms, compute_initial_forces: bool = False):
+        self.L = load()
+        self.params = params
+        self.natom = params.natom
+        self._in = abi.AtomBuffers.from_state(atoms)
+        h = C.c_void_p()
+        rc = self.L.sld_create(C.byref(params), C.byref(self._in.view()),
+                               int(compute_initial_forces), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.sld_last_error(h).decode() if h else "sld_create failed"
+            self.close()
+            raise SldError(rc, msg)
+        n3 = np.zeros(3, dtype=np.int32)
+        ng = np.zeros(1, dtype=np.int32)
+        self._rc(self.L.sld_get_geometry(self.h, n3.ctypes.data_as(_ip), ng.ctypes.data_as(_ip), None))
+        self.ncell3 = tuple(int(x) for x in n3)
+        self.ncells = int(np.prod(n3))
+        self.ngroups = int(ng[0])
+
+    def _rc(self, rc: int):
+        if rc != 0:
+            raise SldError(rc, self.L.sld_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sld_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference entry points
+    def links(self): self._rc(self.L.sld_links(self.h))
+    def calculate_rho(self): self._rc(self.L.sld_calculate_rho(self.h))
+    def calculate_force_energy(self): self._rc(self.L.sld_calculate_force_energy(self.h))
+    def calculate_temperature(self): self._rc(self.L.sld_calculate_temperature(self.h))
+    def core_ds(self, dt): self._rc(self.L.sld_core_ds(self.h, dt))
+    def core_dp(self, dt): self._rc(self.L.sld_core_dp(self.h, dt))
+    def core_dr(self, dt): self._rc(self.L.sld_core_dr(self.h, dt))
+    def core_dTe(self, dt): self._rc(self.L.sld_core_dTe(self.h, dt))
+
+    def check(self, step: int) -> Dict[str, float]:
+        ob = abi.SldObs()
+        self._rc(self.L.sld_check(self.h, step, C.byref(ob)))
+        return abi.obs_dict(ob)
+
+    def step(self, first_step: int, nsteps: int):
+        obs = (abi.SldObs * max(nsteps, 1))()
+        k = C.c_int(0)
+        self._rc(self.L.sld_step(self.h, first_step, nsteps, obs, C.byref(k)))
+        return [abi.obs_dict(obs[i]) for i in range(k.value)]
+
+    def last_step_ms(self) -> float:
+        ms = C.c_double(0.0)
+        self._rc(self.L.sld_last_step_timing(self.h, C.byref(ms)))
+        return ms.value
+
+    def time_kernel(self, name: Optional[str]):
+        """Time every launch of one kernel with HIP events (None: off)."""
+        self._rc(self.L.sld_time_kernel(self.h, (name or "").encode()))
+
+    def kernel_timing(self):
+        """(launches, total device ms) of the timed kernel over the last step call."""
+        n = C.c_int(0)
+        ms = C.c_double(0.0)
+        self._rc(self.L.sld_kernel_timing(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def download(self) -> Dict[str, np.ndarray]:
+        out = abi.AtomBuffers.empty(self.natom)
+        self._rc(self.L.sld_download(self.h, C.byref(out.view())))
+        return out.as_dict()
+
+    def upload(self, st: Dict[str, np.ndarray]):
+        buf = abi.AtomBuffers.from_dict(st, self.natom)
+        self._rc(self.L.sld_upload(self.h, C.byref(buf.view())))
+
+    def cells(self) -> Dict[str, np.ndarray]:
+        out = abi.CellBuffers.empty(self.ncells)
+        self._rc(self.L.sld_get_cells(self.h, C.byref(out.view())))
+        return out.as_dict()
+
+    def set_cells_te(self, te):
+        te = np.ascontiguousarray(te, dtype=np.float64)
+        self._rc(self.L.sld_set_cells_te(self.h, te.ctypes.data_as(_dp)))
+
+    def groups(self):
+        sizes = np.zeros(self.ngroups, dtype=np.int32)
+        self._rc(self.L.sld_get_geometry(self.h, None, None, sizes.ctypes.data_as(_ip)))
+        return sizes
+
+    def cell_lists(self, cap: int = 64):
+        counts = np.zeros(self.ncells, dtype=np.int32)
+        lists = np.full(self.ncells * cap, -1, dtype=np.int32)
+        self._rc(self.L.sld_get_cell_lists(self.h, cap, counts.ctypes.data_as(_ip),
+                                           lists.ctypes.data_as(_ip)))
+        return counts, lists.reshape(self.ncells, cap)
+
+    @property
+    def total_time(self) -> float:
+        return self.L.sld_total_time(self.h)
+
+
+def device_info():
+    L = load()
+    name = C.create_string_buffer(64)
+    ncu = C.c_int(0)
+    rc = L.sld_device_info(name, 64, C.byref(ncu))
+    if rc != 0:
+        raise SldError(rc, "no HIP device")
+    return name.value.decode(), ncu.value

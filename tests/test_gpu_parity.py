@@ -1,0 +1,124 @@
+"""GPU parity: the HIP engine against the plain-C oracle on the same seeded
+inputs, through the C ABI (spilady_amd/engine.py -> libspilady_hip.so).
+
+Both sides draw Langevin noise from the same counter-based Philox stream, so
+thermostatted runs are compared step by step too. Floating-point fields are
+held to a normwise relative error of 1e-10 per array (helpers.TOL); integer
+data (cell lists, groups) must be identical.
+"""
+import numpy as np
+import pytest
+
+from helpers import TOL, assert_close, consistent_state, fe_params, relerr
+from oracle import orcpy
+
+pytestmark = pytest.mark.gpu
+
+ATOM_FIELDS = ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0", "stress")
+CELL_FIELDS = ("te", "tl", "ts_r", "ts_l", "gel", "ges")
+
+
+def _pair(nu, T, seed, **kw):
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(nu, T, seed, **kw)
+    st = consistent_state(p, at)
+    eng = Engine(p, st, compute_initial_forces=False)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_PHILOX)
+    return p, eng, orc
+
+
+def _compare(eng, orc, tag, fields=ATOM_FIELDS, tol=TOL, cells=False):
+    a, b = orc.download(), eng.download()
+    worst = {}
+    for k in fields:
+        worst[k] = assert_close(a[k], b[k], f"{tag}:{k}", tol)
+    if cells:
+        ca, cb = orc.cells(), eng.cells()
+        for k in CELL_FIELDS:
+            worst["cell_" + k] = assert_close(ca[k], cb[k], f"{tag}:cell {k}", tol)
+    return worst
+
+
+def test_initial_force_energy():
+    """calculate_rho + calculate_force_energy from the same positions and
+    spins (no thermostats)."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(10, 300.0, 1, thermostats=False)
+    st = consistent_state(p, at)
+    eng = Engine(p, at, compute_initial_forces=True)
+    b = eng.download()
+    for k in ("rho", "f", "pe", "me", "me0", "stress"):
+        assert_close(st[k], b[k], k)
+    eng.close()
+
+
+def test_cell_geometry_and_groups():
+    p, eng, orc = _pair(8, 300.0, 3, thermostats=False)
+    assert eng.ncell3 == orc.ncell3
+    sizes, _ = orc.groups()
+    np.testing.assert_array_equal(eng.groups(), sizes)
+    c1, l1 = orc.cell_lists(32)
+    c2, l2 = eng.cell_lists(32)
+    np.testing.assert_array_equal(c1, c2)
+    np.testing.assert_array_equal(l1, l2)
+    eng.close(); orc.close()
+
+
+@pytest.mark.parametrize("nu", [10, 16])
+def test_entry_points_one_by_one_full_physics(nu):
+    """Every reference entry point in core() order, thermostats, eltemp and
+    local collective motion on; 10^3 unit cells (6 link cells per side,
+    8 groups) and 16^3 (11 per side, 18 groups)."""
+    p, eng, orc = _pair(nu, 600.0, 7, thermostats=True, eltemp=True, localcolmot=True,
+                        renormalize=False)
+    dt = p.step
+    seq = [("links", ()), ("core_dTe", (dt / 2,)), ("core_dp", (dt / 2,)), ("core_ds", (dt / 2,)),
+           ("core_dr", (dt,)), ("calculate_rho", ()), ("core_ds", (dt / 2,)),
+           ("calculate_force_energy", ()), ("core_dp", (dt / 2,)),
+           ("calculate_temperature", ()), ("core_dTe", (dt / 2,))]
+    for name, args in seq:
+        getattr(orc, name)(*args)
+        getattr(eng, name)(*args)
+        _compare(eng, orc, name, cells=True)
+    eng.close(); orc.close()
+
+
+def test_steps_nve():
+    """Thermostats off: 20 full steps, observables of every step and the
+    final state."""
+    p, eng, orc = _pair(10, 300.0, 1, thermostats=False)
+    oa = orc.step(0, 20)
+    ob = eng.step(0, 20)
+    assert len(oa) == len(ob) == 20
+    for x, y in zip(oa, ob):
+        for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0", "tl", "ts_r", "ts_l", "ave_s0", "virial"):
+            assert abs(x[k] - y[k]) <= TOL * max(abs(x[k]), 1e-300), (x["step"], k, x[k], y[k])
+    _compare(eng, orc, "nve-20")
+    eng.close(); orc.close()
+
+
+def test_steps_langevin_eltemp_renormalize():
+    """All switches on, renormalizeEnergy included, 5 steps (the reference's
+    own Te stays finite that long from this start)."""
+    p, eng, orc = _pair(10, 300.0, 1, thermostats=True, eltemp=True, localcolmot=True,
+                        renormalize=True)
+    oa = orc.step(0, 5)
+    ob = eng.step(0, 5)
+    for x, y in zip(oa, ob):
+        for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0", "ave_ee", "tl", "ts_l", "te"):
+            assert abs(x[k] - y[k]) <= 1e-9 * max(abs(x[k]), 1e-300), (x["step"], k, x[k], y[k])
+    _compare(eng, orc, "lang-5", cells=True, tol=1e-9)
+    eng.close(); orc.close()
+
+
+def test_cell_lists_after_movers():
+    """Hot lattice (1400 K) for 30 steps: atoms change link cells and the
+    per-cell list order must follow the reference's relink rule exactly."""
+    p, eng, orc = _pair(8, 1400.0, 11, thermostats=True)
+    orc.step(0, 30)
+    eng.step(0, 30)
+    c1, l1 = orc.cell_lists(32)
+    c2, l2 = eng.cell_lists(32)
+    np.testing.assert_array_equal(c1, c2)
+    np.testing.assert_array_equal(l1, l2)
+    eng.close(); orc.close()
