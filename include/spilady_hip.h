@@ -96,6 +96,15 @@ typedef struct sld_ctx sld_ctx;
 
 /* Library identification: ABI version and the device it bound to. */
 int sld_version(void);
+/* Host-only planning, no device call: link-cell counts for a box
+ * (map_cells.cpp:41-60), the greedy cell groups (allocate_cells_CPU.cpp:45-122)
+ * and the dependency level of every cell in the four group-ordered sweeps the
+ * engine launches level by level (spin forward/backward over the 26
+ * neighbours, electron diffusion forward/backward over the 6 faces).
+ * Call with group_sizes == NULL to get ncell3 only. Arrays hold C =
+ * ncell3[0]*ncell3[1]*ncell3[2] entries (levels: 4*C, schedule-major). */
+int sld_plan(const double* box, double min_length_link_cell, int* ncell3, int* ngroups,
+             int* group_sizes, int* group_cells, int* nlevels, int* levels);
 int sld_device_info(char* name, int len, int* n_cu);
 
 /* Replaces initial_GPU + potential_table_GPU + copy_CPU_to_GPU +

@@ -206,7 +206,7 @@ void greedy_groups(const HostCells& h, std::vector<int>& gsize, std::vector<int>
 // reproduces the sweep exactly.
 void make_levels(const HostCells& h, const std::vector<int>& gsize, const std::vector<int>& gcells,
                  const std::vector<int>& group_of, const int* slots, int nslots, bool backward,
-                 LevelSet& L) {
+                 LevelSet& L, std::vector<int>* level_out = nullptr) {
     std::vector<int> level(h.C, -1);
     int G = (int)gsize.size();
     std::vector<int> goff(G + 1, 0);
@@ -229,6 +229,7 @@ void make_levels(const HostCells& h, const std::vector<int>& gsize, const std::v
             maxl = std::max(maxl, lv);
         }
     }
+    if (level_out) *level_out = level;
     L.off.assign(maxl + 2, 0);
     for (int c = 0; c < h.C; ++c) L.off[level[c] + 1]++;
     for (int l = 0; l <= maxl; ++l) L.off[l + 1] += L.off[l];
@@ -700,6 +701,33 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
 extern "C" {
 
 int sld_version(void) { return SLD_ABI_VERSION; }
+
+int sld_plan(const double* box, double min_length_link_cell, int* ncell3, int* ngroups,
+             int* group_sizes, int* group_cells, int* nlevels, int* levels) {
+    if (!box || !ncell3) return SLD_ERR_ARG;
+    HostCells hc = cell_counts(box, min_length_link_cell);
+    ncell3[0] = hc.nx; ncell3[1] = hc.ny; ncell3[2] = hc.nz;
+    if (!group_sizes) return SLD_OK;
+    if (hc.nx < 3 || hc.ny < 3 || hc.nz < 3) return SLD_ERR_ARG;
+    std::vector<int> gs, gc, gof;
+    greedy_groups(hc, gs, gc, gof);
+    if (ngroups) *ngroups = (int)gs.size();
+    std::copy(gs.begin(), gs.end(), group_sizes);
+    if (group_cells) std::copy(gc.begin(), gc.end(), group_cells);
+    int all26[26];
+    for (int k = 0; k < 26; ++k) all26[k] = k;
+    const int faces[6] = {0, 13, 2, 14, 12, 15};
+    for (int sch = 0; sch < 4; ++sch) {
+        LevelSet L;
+        std::vector<int> lv;
+        bool bwd = sch % 2 == 1;
+        if (sch < 2) make_levels(hc, gs, gc, gof, all26, 26, bwd, L, &lv);
+        else make_levels(hc, gs, gc, gof, faces, 6, bwd, L, &lv);
+        if (nlevels) nlevels[sch] = L.nlev();
+        if (levels) std::copy(lv.begin(), lv.end(), levels + (size_t)sch * hc.C);
+    }
+    return SLD_OK;
+}
 
 int sld_device_info(char* name, int len, int* n_cu) {
     int dev = 0;

@@ -53,6 +53,7 @@ def load(path: Optional[str] = None):
     A = C.POINTER(abi.SldAtoms)
     V = C.c_void_p
     L.sld_version.restype = C.c_int
+    L.sld_plan.argtypes = [_dp, C.c_double, _ip, _ip, _ip, _ip, _ip, _ip]
     L.sld_device_info.argtypes = [C.c_char_p, C.c_int, _ip]
     L.sld_create.argtypes = [P, A, C.c_int, C.POINTER(V)]
     L.sld_destroy.argtypes = [V]
@@ -190,6 +191,32 @@ class Engine(abi.EngineLike):
     @property
     def total_time(self) -> float:
         return self.L.sld_total_time(self.h)
+
+
+def plan(box, min_length_link_cell: float):
+    """Host-only link-cell plan (no device call): cell counts, greedy groups
+    and the per-cell levels of the four level-scheduled sweeps."""
+    L = load()
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    n3 = np.zeros(3, dtype=np.int32)
+    rc = L.sld_plan(b.ctypes.data_as(_dp), min_length_link_cell, n3.ctypes.data_as(_ip),
+                    None, None, None, None, None)
+    if rc != 0:
+        raise SldError(rc, "sld_plan")
+    C_ = int(np.prod(n3))
+    ng = np.zeros(1, dtype=np.int32)
+    gs = np.zeros(C_, dtype=np.int32)
+    gc = np.zeros(C_, dtype=np.int32)
+    nl = np.zeros(4, dtype=np.int32)
+    lv = np.zeros(4 * C_, dtype=np.int32)
+    rc = L.sld_plan(b.ctypes.data_as(_dp), min_length_link_cell, n3.ctypes.data_as(_ip),
+                    ng.ctypes.data_as(_ip), gs.ctypes.data_as(_ip), gc.ctypes.data_as(_ip),
+                    nl.ctypes.data_as(_ip), lv.ctypes.data_as(_ip))
+    if rc != 0:
+        raise SldError(rc, "sld_plan")
+    g = int(ng[0])
+    return {"ncell3": tuple(int(x) for x in n3), "group_sizes": gs[:g].copy(), "group_cells": gc,
+            "nlevels": nl, "levels": lv.reshape(4, C_)}
 
 
 def device_info():

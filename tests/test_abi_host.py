@@ -1,0 +1,84 @@
+"""The C ABI library without a GPU: it loads, exports every entry point that
+include/spilady_hip.h declares, and its host-only planning (link cells,
+groups, sweep levels) agrees with the reference's geometry and ordering."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import orcpy
+from spilady_amd import abi, configs, engine
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.load()
+    names = engine.exported_symbols()
+    assert "sld_create" in names and "sld_step" in names and "sld_plan" in names
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    out = subprocess.run(["nm", "-D", "--defined-only", engine.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(names) <= exported
+    assert lib.sld_version() == 1
+
+
+def test_ctypes_structs_match_header_sizes():
+    """Field layout of the plain-data structs: sizes fixed by the header."""
+    assert C.sizeof(abi.SldParams) == 4 * 2 + 8 * 6 + 8 * 11 + 8 + 4 * 6 + 4 * 8
+    assert C.sizeof(abi.SldAtoms) == 14 * 8
+    assert C.sizeof(abi.SldObs) == 8 + 8 + 8 * 5 + 8 * 4 + 8 * 8 + 8
+
+
+def test_plan_matches_reference_groups():
+    g = np.load(os.path.join(GOLD, "groups.npz"), allow_pickle=False)
+    for nu in g["nus"]:
+        L = int(nu) * 2.8665
+        pl = engine.plan([L, 0, L, 0, 0, L], 4.11)
+        assert pl["ncell3"] == tuple(g[f"nu{nu}_ncell3"])
+        np.testing.assert_array_equal(pl["group_sizes"], g[f"nu{nu}_sizes"])
+        np.testing.assert_array_equal(pl["group_cells"], g[f"nu{nu}_members"])
+
+
+@pytest.mark.parametrize("nu", [8, 10, 16])
+def test_sweep_levels_reproduce_group_order(nu):
+    """A level schedule is exact if (a) cells of one level never touch each
+    other through the stencil of the sweep and (b) every pair of touching
+    cells is ordered as the groups order them (forward: lower group first;
+    backward: higher group first)."""
+    L = nu * 2.8665
+    pl = engine.plan([L, 0, L, 0, 0, L], 4.11)
+    nx, ny, nz = pl["ncell3"]
+    C_ = nx * ny * nz
+    group_of = np.empty(C_, dtype=int)
+    off = np.concatenate([[0], np.cumsum(pl["group_sizes"])])
+    for gi in range(len(pl["group_sizes"])):
+        group_of[pl["group_cells"][off[gi]:off[gi + 1]]] = gi
+    cells = np.arange(C_)
+    x, y, z = cells % nx, (cells // nx) % ny, cells // (nx * ny)
+    offs26 = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)
+              if (dx, dy, dz) != (0, 0, 0)]
+    offs6 = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for sch, offs in enumerate((offs26, offs26, offs6, offs6)):
+        lv = pl["levels"][sch]
+        assert lv.max() + 1 == pl["nlevels"][sch]
+        backward = sch % 2 == 1
+        for dx, dy, dz in offs:
+            nb = ((x + dx) % nx) + ((y + dy) % ny) * nx + ((z + dz) % nz) * nx * ny
+            assert np.all(lv != lv[nb]), (sch, dx, dy, dz)
+            earlier = group_of[nb] > group_of if backward else group_of[nb] < group_of
+            assert np.all(lv[nb][earlier] < lv[earlier]), (sch, dx, dy, dz)
+
+
+def test_plan_levels_even_box_merge_face_sweep():
+    """With an even number of link cells per side the 8 groups form the 2x2x2
+    parity pattern: the face-only electron sweep needs 4 levels, the spin
+    sweep 8."""
+    L = 10 * 2.8665
+    pl = engine.plan([L, 0, L, 0, 0, L], 4.11)
+    assert pl["ncell3"] == (6, 6, 6)
+    assert list(pl["nlevels"]) == [8, 8, 4, 4]

@@ -1,0 +1,74 @@
+"""SPILADY file formats and the seeded bcc Fe generator."""
+import os
+
+import numpy as np
+
+from spilady_amd import abi, configs, io
+from spilady_amd import constants as K
+
+
+def test_bcc100_order_and_offset():
+    """bcc100.cpp:55-87: for z, y, x unit cells the corner then the body
+    centre, everything shifted by a/4."""
+    a = 2.0
+    r = configs.bcc100_positions(3, 2, 2, a)
+    assert r.shape == (24, 3)
+    np.testing.assert_allclose(r[0], [0.5, 0.5, 0.5])
+    np.testing.assert_allclose(r[1], [1.5, 1.5, 1.5])
+    np.testing.assert_allclose(r[2], [2.5, 0.5, 0.5])          # next x cell
+    np.testing.assert_allclose(r[6], [0.5, 2.5, 0.5])          # next y row
+    np.testing.assert_allclose(r[12], [0.5, 0.5, 2.5])         # next z layer
+    q = r / np.array([3 * a, 2 * a, 2 * a])
+    assert q.min() >= 0.25 / 3 - 1e-12 and q.max() < 1.0
+
+
+def test_generator_statistics():
+    at = configs.make_bcc_fe(12, 600.0, 7)
+    m = K.mass_internal(K.FE_MASS_U)
+    np.testing.assert_allclose(at.p.sum(axis=0), 0.0, atol=1e-25)
+    t = (at.p ** 2).sum() / (3 * at.natom) / m / K.BOLTZ
+    assert abs(t - 600.0) < 600.0 * 0.05
+    s0 = np.linalg.norm(at.s, axis=1)
+    np.testing.assert_allclose(s0, 2.2 / K.EL_G, rtol=1e-14)
+    tilt = np.degrees(np.arccos(at.s[:, 2] / s0))
+    assert tilt.max() <= 10.0 and tilt.min() >= 0.0
+    # seeded: same seed, same atoms
+    np.testing.assert_array_equal(configs.make_bcc_fe(12, 600.0, 7).p, at.p)
+
+
+def test_con_file_round_trip(tmp_path):
+    at = configs.make_bcc_fe(3, 300.0, 2)
+    at.f[:] = np.random.default_rng(0).normal(size=at.f.shape)
+    at.heff_h = np.random.default_rng(1).normal(size=(at.natom, 3))
+    path = tmp_path / "con-test_0000.dat"
+    io.write_con(str(path), at)
+    back = io.read_con(str(path))
+    for k in ("r", "p", "f", "s", "stress", "rho", "ke", "pe", "me", "me0"):
+        np.testing.assert_allclose(getattr(back, k), getattr(at, k), rtol=1e-15, atol=0)
+    np.testing.assert_array_equal(back.box, at.box)
+    # the Heff y column repeats x (write_config.cpp:117)
+    np.testing.assert_allclose(back.heff_h[:, 1], at.heff_h[:, 0], rtol=1e-15)
+    with open(path) as fh:
+        head = fh.readline().split()
+    assert head[0] == str(at.natom) and head[2] == "SLDHL"
+    assert io.con_name("run", 3) == "con-run_0003.dat"
+
+
+def test_variables_round_trip(tmp_path):
+    v = configs.fe_variables(450.0, 17, thermostats=True, eltemp=True)
+    path = tmp_path / "variables.in"
+    io.write_variables(str(path), v)
+    back = io.read_variables(str(path))
+    for k, x in v.items():
+        assert back[k] == x, k
+    p = abi.params_from_variables(back, [10, 0, 10, 0, 0, 10], 100)
+    assert abs(p.temperature - 450.0 * K.BOLTZ) < 1e-18
+    assert p.eltemp == 1 and p.spinlang == 1 and p.random_seed == 17
+
+
+def test_min_length_link_cell_raised_to_rcut():
+    """read_variables.cpp:262: rcut_max above min_length_link_cell raises it."""
+    v = configs.fe_variables(300.0, 1, thermostats=False)
+    v["min_length_link_cell"] = 3.0
+    p = abi.params_from_variables(v, [10, 0, 10, 0, 0, 10], 10)
+    assert abs(p.min_length_link_cell - (4.1001 + 0.001)) < 1e-12

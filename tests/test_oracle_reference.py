@@ -1,0 +1,158 @@
+"""The plain-C oracle against the compiled SPILADY reference.
+
+Fixtures in tests/golden/ were written by tests/golden/make_golden.py from the
+reference built out of its own sources (oracle/Makefile.ref). The oracle runs
+the reference's loop order on one thread with the reference's MWC noise
+stream, so agreement is bit for bit. When oracle/_ref is present the same
+comparison also runs live.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import orcpy, refpy
+from spilady_amd import abi, configs
+from spilady_amd.io import AtomState
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def _variables(g):
+    keys = [str(k) for k in g["variables_keys"]]
+    vals = [str(v) for v in g["variables_values"]]
+    out = {}
+    for k, v in zip(keys, vals):
+        try:
+            out[k] = int(v)
+        except ValueError:
+            try:
+                out[k] = float(v)
+            except ValueError:
+                out[k] = v
+    return out
+
+
+def _oracle_from_fixture(g):
+    v = _variables(g)
+    n = g["in_r"].shape[0]
+    st = {k: g[f"in_{k}"] for k in ("r", "p", "f", "s", "stress", "rho", "ke", "pe", "me", "me0")}
+    # the reference build has every control.h switch on (refpy.REF_SWITCHES)
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), g["box"], n)
+    return v, p, orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_MWC)
+
+
+def test_tables_match_reference():
+    g = _load("tables.npz")
+    at = configs.make_bcc_fe(6, 300.0, 1)
+    v = configs.fe_variables(300.0, 1, thermostats=False)
+    p = abi.params_from_variables(v, at.box, at.natom)
+    o = orcpy.Oracle(p, at, compute_initial_forces=True)
+    for w in range(16):
+        xs = g["x_rho"] if w in (0, 1) or w >= 8 else g["x_r"]
+        got = np.array([o.table(w, float(x)) for x in xs])
+        np.testing.assert_array_equal(got, g[f"t{w}"], err_msg=f"table {w}")
+        raw = np.array([o.table_raw(w, int(i)) for i in g["raw_index"]])
+        np.testing.assert_array_equal(raw, g["raw"][w], err_msg=f"raw table {w}")
+    # density of the perfect lattice through the reference's own evaluation
+    np.testing.assert_array_equal(o.download()["rho"], g["perfect_rho"])
+    o.close()
+
+
+def test_groups_match_reference():
+    g = _load("groups.npz")
+    for nu in g["nus"]:
+        at = configs.make_bcc_fe(int(nu), 300.0, 1)
+        v = configs.fe_variables(300.0, 1, thermostats=False)
+        p = abi.params_from_variables(v, at.box, at.natom)
+        at.rho[:] = 0.48
+        o = orcpy.Oracle(p, at)
+        assert o.ncell3 == tuple(g[f"nu{nu}_ncell3"])
+        sizes, members = o.groups()
+        np.testing.assert_array_equal(sizes, g[f"nu{nu}_sizes"])
+        np.testing.assert_array_equal(members, g[f"nu{nu}_members"])
+        o.close()
+
+
+@pytest.mark.parametrize("name", ["step_nve.npz", "step_lang.npz"])
+def test_entry_points_match_reference(name):
+    """Each reference entry point of one core() step, bit for bit."""
+    g = _load(name)
+    v, p, o = _oracle_from_fixture(g)
+    dt = p.step
+    args = {"core_dTe": (dt / 2,), "core_dp": (dt / 2,), "core_ds": (dt / 2,), "core_dr": (dt,)}
+    for i, nm in enumerate(str(x) for x in g["seq"]):
+        getattr(o, nm)(*args.get(nm, ()))
+        a, c = o.download(), o.cells()
+        keys = [k[len(f"s{i}_"):] for k in g.files if k.startswith(f"s{i}_")]
+        for k in keys:
+            ref = g[f"s{i}_{k}"]
+            if k == "cell_counts":
+                got = o.cell_lists(32)[0]
+            elif k == "cell_lists":
+                got = o.cell_lists(32)[1]
+            elif k.startswith("cell_"):
+                got = c[k[5:]]
+            else:
+                got = a[k]
+            np.testing.assert_array_equal(got, ref, err_msg=f"{name} step {i} ({nm}) field {k}")
+    o.close()
+
+
+@pytest.mark.parametrize("name", ["step_nve.npz", "step_lang.npz"])
+def test_trajectory_matches_reference(name):
+    """nsteps of core() + check() (renormalizeEnergy on): the enr-/tmp-/spn-
+    columns as printed (%.15e) and the final state, bit for bit."""
+    g = _load(name)
+    v, p, o = _oracle_from_fixture(g)
+    n = int(g["nsteps"])
+    obs = o.step(0, n)
+    enr = np.array([[x["ave_pe"], x["ave_ke"], x["ave_me"], x["ave_me0"], x["ave_ee"]] for x in obs])
+    tmp = np.array([[x["tl"], x["ts_r"], x["ts_l"], x["te"]] for x in obs])
+    spn = np.array([list(x["ave_s"]) + [x["ave_s0"]] + list(x["ave_m"]) + [x["ave_m0"]] for x in obs])
+    as_printed = lambda a: np.array([[float("%.15e" % y) for y in row] for row in a])  # noqa: E731
+    np.testing.assert_array_equal(as_printed(enr), g["enr"][1:, 2:])
+    np.testing.assert_array_equal(as_printed(tmp), g["tmp"][1:, 2:])
+    np.testing.assert_array_equal(as_printed(spn), g["spn"][1:, 2:])
+    a = o.download()
+    for k in ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0"):
+        np.testing.assert_array_equal(a[k], g[f"final_{k}"], err_msg=k)
+    np.testing.assert_array_equal(o.cells()["te"], g["final_cell_te"])
+    counts, lists = o.cell_lists(32)
+    np.testing.assert_array_equal(counts, g["final_cell_counts"])
+    np.testing.assert_array_equal(lists, g["final_cell_lists"])
+    o.close()
+
+
+@pytest.mark.skipif(not refpy.available(), reason="compiled reference not built (oracle/Makefile.ref)")
+def test_live_reference_agrees_hot_lattice():
+    """Live run of the compiled reference against the oracle on a hotter,
+    displaced lattice (movers between link cells), thermostats on."""
+    at = configs.make_bcc_fe(8, 900.0, 5, displace_sigma=0.05, tilt_max_deg=30.0)
+    v = configs.fe_variables(900.0, 5, thermostats=True, eltemp=True, localcolmot=True,
+                             renormalize=False)
+    st = refpy.consistent_start(v, at)
+    d = tempfile.mkdtemp()
+    ref = refpy.Reference()
+    ref.init_memory(d, v, st)
+    ref.run(0, 8, False)
+    a = ref.atoms()
+    counts_r, lists_r = ref.cell_lists(32)
+    ref.free()
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), st.box, st.natom)
+    p.renormalize_energy = 0
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    for s in range(8):
+        o.core(s)
+    b = o.download()
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    counts_o, lists_o = o.cell_lists(32)
+    np.testing.assert_array_equal(counts_r, counts_o)
+    np.testing.assert_array_equal(lists_r, lists_o)
+    o.close()
