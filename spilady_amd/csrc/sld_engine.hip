@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -27,6 +28,7 @@ struct LevelSet {
     std::vector<int> cells;   // cells grouped by level
     std::vector<int> off;     // level offsets (size nlev + 1)
     int* d_cells = nullptr;
+    int* d_off = nullptr;
     int nlev() const { return (int)off.size() - 1; }
 };
 
@@ -57,6 +59,7 @@ struct sld_ctx {
     bool nb_dirty = true;
     double vol = 0.0, dx2 = 0, dy2 = 0, dz2 = 0;
     double last_ms = 0.0;
+    // persistent core_dTe
     // per-kernel event timing of one named kernel
     std::string timed_name;
     std::vector<hipEvent_t> ev_pool;
@@ -248,13 +251,13 @@ void make_levels(const HostCells& h, const std::vector<int>& gsize, const std::v
 // ---------------------------------------------------------------------------
 // device building blocks
 
-int reduce(sld_ctx* c, int nblocks, int nk, double* out) {
-    LAUNCH(c, k_reduce, 1, 256, c->d_part, nblocks, nk, out);
+int reduce(sld_ctx* c, int nblocks, int nk, double* out, size_t part_offset = 0) {
+    LAUNCH(c, k_reduce, 1, 256, (const double*)(c->d_part + part_offset), nblocks, nk, out);
     return SLD_OK;
 }
 
 int nbuild(sld_ctx* c, int write_rho) {
-    LAUNCH(c, k_nbuild, nblk(c->D.N, 256), 256, c->box, c->tab, c->phys, c->D, write_rho);
+    LAUNCH(c, k_nbuild, nblk(c->D.C, 4), 256, c->box, c->tab, c->phys, c->D, write_rho);
     c->nb_dirty = false;
     return SLD_OK;
 }
@@ -287,7 +290,7 @@ int do_ds(sld_ctx* c, double dt) {
         LevelSet& L = dir == 0 ? c->spin_fwd : c->spin_bwd;
         for (int l = 0; l < L.nlev(); ++l) {
             int n = L.off[l + 1] - L.off[l];
-            LAUNCH(c, k_spin_phase, nblk(n, 64), 64, L.d_cells + L.off[l], n, dir, dth, c->tab,
+            LAUNCH(c, k_spin_phase, nblk(n, 128), 128, L.d_cells + L.off[l], n, dir, dth, c->tab,
                    c->phys, c->D, c->pass);
         }
         c->pass++;
@@ -296,7 +299,7 @@ int do_ds(sld_ctx* c, double dt) {
 }
 
 int do_dp(sld_ctx* c, double dt) {
-    int nb = nblk(c->D.C, 256);
+    int nb = nblk(c->D.C, 8);
     LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, c->pass, c->d_part);
     if (c->phys.lattlang) c->pass++;
     TRY(reduce(c, nb, 3, c->d_scal));
@@ -315,7 +318,7 @@ int do_dr(sld_ctx* c, double dt) {
 
 int do_temperature(sld_ctx* c) {
     TRY(ensure_nb(c));
-    LAUNCH(c, k_temperature, nblk(c->D.C, 128), 128, c->tab, c->phys, c->D);
+    LAUNCH(c, k_temperature, nblk(c->D.C, 8), 256, c->tab, c->phys, c->D);
     return SLD_OK;
 }
 
@@ -329,20 +332,23 @@ int do_dTe(sld_ctx* c, double dt) {
     double vf = atomic_volume / ((double)c->D.N / (double)c->D.C);
     double pref = dtl / 2.0 * kappa * vf / kBoltz;
     int nb = nblk(c->D.C, 256);
+    LAUNCH(c, k_dte_ab, nb, 256, h, c->phys, c->D, c->d_part);
     for (int it = 0; it < nloop; ++it) {
-        LAUNCH(c, k_dte_ab, nb, 256, h, c->phys, c->D, c->d_part);
         TRY(reduce(c, nb, 1, c->d_scal + 8));
         for (int dir = 0; dir < 2; ++dir) {
             LevelSet& L = dir == 0 ? c->dte_fwd : c->dte_bwd;
             for (int l = 0; l < L.nlev(); ++l) {
                 int n = L.off[l + 1] - L.off[l];
-                LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, c->box, pref,
-                       c->dx2, c->dy2, c->dz2, c->D);
+                LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, pref, c->dx2,
+                       c->dy2, c->dz2, c->D);
             }
         }
-        LAUNCH(c, k_dte_end, nb, 256, c->D, c->d_part);
-        TRY(reduce(c, nb, 1, c->d_scal + 9));
-        LAUNCH(c, k_dte_finish, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8));
+        LAUNCH(c, k_dte_end, nb, 256, c->D, c->d_part + (size_t)nb);
+        TRY(reduce(c, nb, 1, c->d_scal + 9, (size_t)nb));
+        if (it + 1 < nloop)
+            LAUNCH(c, k_dte_finish_ab, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8), 1, c->d_part);
+        else
+            LAUNCH(c, k_dte_finish, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8), 1);
     }
     return SLD_OK;
 }
@@ -497,7 +503,10 @@ int down1(sld_ctx* c, const double* d, double* dst, size_t n) {
 
 int upload_levels(sld_ctx* c, LevelSet& L) {
     TRY(dalloc(c, &L.d_cells, L.cells.size()));
+    TRY(dalloc(c, &L.d_off, L.off.size()));
     HIPCHK(c, hipMemcpyAsync(L.d_cells, L.cells.data(), L.cells.size() * sizeof(int),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(L.d_off, L.off.data(), L.off.size() * sizeof(int),
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SLD_OK;
@@ -584,7 +593,9 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
     TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
     TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
-    TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N));
+    TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
+    TRY(dalloc(c, &D.fn6, (size_t)6 * C));
+
     TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
     TRY(dalloc(c, &c->d_cl2, (size_t)C * kCellCap)); TRY(dalloc(c, &c->d_cn2, C));
     TRY(dalloc(c, &D.nb, (size_t)N * kMaxNb)); TRY(dalloc(c, &D.nbn, N));
@@ -595,7 +606,7 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
     TRY(dalloc(c, &D.TsL, C)); TRY(dalloc(c, &D.sRup, C)); TRY(dalloc(c, &D.sRdn, C));
     TRY(dalloc(c, &D.sLup, C)); TRY(dalloc(c, &D.sLdn, C)); TRY(dalloc(c, &D.cellv, C));
     TRY(dalloc(c, &D.err, 1));
-    int maxblk = std::max(nblk(N, 256), nblk(C, 128)) + 1;
+    int maxblk = std::max(nblk(N, 256), nblk(C, 8)) + 1;
     TRY(dalloc(c, &c->d_part, (size_t)maxblk * 16));
     TRY(dalloc(c, &c->d_scal, 64));
     TRY(dalloc(c, &c->d_einit, 4));
@@ -655,6 +666,20 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
     HIPCHK(c, hipMemcpyAsync(D.cell_prev, cell.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.cn, cn.data(), C * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.cl, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    std::vector<int> cslot(N);
+    for (int cc = 0; cc < C; ++cc)
+        for (int q = 0; q < cn[cc]; ++q) cslot[cl[(size_t)cc * kCellCap + q]] = cc * kCellCap + q;
+    std::vector<int> fn6((size_t)6 * C);
+    {
+        int nbh[26];
+        const int faces[6] = {0, 13, 2, 14, 12, 15};   // +x -x +y -y +z -z (map_cells.cpp:96-121)
+        for (int cc = 0; cc < C; ++cc) {
+            host_neigh(hc, cc, nbh);
+            for (int k = 0; k < 6; ++k) fn6[(size_t)6 * cc + k] = nbh[faces[k]];
+        }
+    }
+    HIPCHK(c, hipMemcpyAsync(D.cslot, cslot.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.fn6, fn6.data(), fn6.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
 
     // groups and sweep levels

@@ -17,6 +17,8 @@ struct Dev {
     double* stress;            // [6][N]: 11 22 33 12 23 31
     int* cell;                 // link cell of each atom (new_cell_index)
     int* cell_prev;            // link cell before the last links()
+    int* cslot;                // cell * kCellCap + position in the cell's list
+    int* fn6;                  // face neighbours of each cell [C][6]: +x -x +y -y +z -z
     int *cl, *cn;              // cell lists [C][kCellCap] head->tail, counts [C]
     int *nb, *nbn;             // neighbours within rcut_max [N][kMaxNb]
     int *nbm, *nbmn;           // neighbours within rcut_mag [N][kMaxMag]
@@ -36,6 +38,8 @@ struct Phys {
     int spinlang, lattlang, eltemp, localcolmot;
 };
 
+// Arguments of the persistent core_dTe kernel.
+
 // ---- kernels ----
 __global__ void k_bin(Box b, Dev D);
 __global__ void k_relink(Box b, const int* cl_old, const int* cn_old, Dev D);
@@ -50,10 +54,11 @@ __global__ void k_dp_scale(Phys P, Dev D, const double* ke2);
 __global__ void k_dr(double dt, Box b, Phys P, Dev D);
 __global__ void k_temperature(Tables T, Phys P, Dev D);
 __global__ void k_dte_ab(double h, Phys P, Dev D, double* partials);
-__global__ void k_dte_sweep(const int* cells, int ncell, Box b, double pref, double dx2, double dy2,
+__global__ void k_dte_sweep(const int* cells, int ncell, double pref, double dx2, double dy2,
                             double dz2, Dev D);
 __global__ void k_dte_end(Dev D, double* partials);
-__global__ void k_dte_finish(double h, Phys P, Dev D, const double* se);
+__global__ void k_dte_finish(double h, Phys P, Dev D, const double* xp, int nxp);
+__global__ void k_dte_finish_ab(double h, Phys P, Dev D, const double* xp, int nxp, double* partials);
 __global__ void k_obs_atoms(Tables T, Phys P, Dev D, double* partials, int scan_spin);
 __global__ void k_obs_cells(Dev D, double* partials);
 __global__ void k_renorm(Dev D, const double* err, double* partials);

@@ -103,13 +103,20 @@ __global__ __launch_bounds__(128) void k_relink(Box b, const int* cl_old, const 
     }
     int* out = D.cl + c * kCellCap;
     int k = 0;
-    for (int q = 0; q < nm; ++q) out[k++] = mv[q];
+    for (int q = 0; q < nm; ++q) {
+        D.cslot[mv[q]] = c * kCellCap + k;
+        out[k++] = mv[q];
+    }
     int n = cn_old[c];
     for (int q = 0; q < n; ++q) {
         int a = cl_old[c * kCellCap + q];
         if (D.cell[a] == c) {
-            if (k < kCellCap) out[k++] = a;
-            else over = true;
+            if (k < kCellCap) {
+                D.cslot[a] = c * kCellCap + k;
+                out[k++] = a;
+            } else {
+                over = true;
+            }
         }
     }
     D.cn[c] = k;
@@ -119,50 +126,120 @@ __global__ __launch_bounds__(128) void k_relink(Box b, const int* cl_old, const 
 // ---------------------------------------------------------------------------
 // neighbour list + density (calculate_rho_CPU.cpp:33-92 for rho, the 27-cell
 // scan order of calculate_spin_CPU.cpp:201-235 for the lists)
+//
+// One wavefront per link cell. The ids of the 27 cells' atoms (slots 0..25,
+// then the cell itself, each in list order) are staged in LDS; every lane
+// holds up to kCandReg candidate positions in registers and the wave walks
+// the cell's atoms, testing all candidates at once. In-range pairs are
+// compacted with a ballot and a lane prefix count, so each list keeps the
+// reference's scan order.
+
+constexpr int kNbWaves = 4;                 // waves (cells) per block
+constexpr int kMaxCand = 27 * kCellCap;     // upper bound of candidates
+constexpr int kCandReg = 4;                 // passes of 64 candidates held in registers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
 
 __global__ __launch_bounds__(256) void k_nbuild(Box b, Tables T, Phys P, Dev D, int write_rho) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= D.N) return;
-    const int ci = D.cell[i];
-    const double3 ri = d3(D.r[i]);
-    int nn = 0, nm = 0;
-    double rho = 0.0;
+    __shared__ int s_cand[kNbWaves][kMaxCand];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int c = blockIdx.x * kNbWaves + w;
+    const bool valid = c < D.C;
+    int cid = 0, cnt = 0;
+    if (valid && lane < 27) {
+        cid = lane < 26 ? neigh_cell(b, c, lane) : c;
+        cnt = D.cn[cid];
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, 26, 64);
+    if (valid && lane < 27) {
+        const int* src = D.cl + cid * kCellCap;
+        const int o = incl - cnt;
+        for (int k = 0; k < cnt; ++k) s_cand[w][o + k] = src[k];
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int npass = (total + 63) >> 6;
+    int jc[kCandReg];
+    double4 rc[kCandReg];
+#pragma unroll
+    for (int q = 0; q < kCandReg; ++q) {
+        int idx = q * 64 + lane;
+        jc[q] = (q < npass && idx < total) ? s_cand[w][idx] : -1;
+        rc[q] = jc[q] >= 0 ? D.r[jc[q]] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int nc = D.cn[c];
+    const int* own = D.cl + c * kCellCap;
     bool over = false;
-    int* nb = D.nb + (size_t)i * kMaxNb;
-    int* nbm = D.nbm + (size_t)i * kMaxMag;
-    double* nbJ = D.nbJ + (size_t)i * kMaxMag;
-    for (int slot = 0; slot <= 26; ++slot) {
-        int wc = slot < 26 ? neigh_cell(b, ci, slot) : ci;
-        int n = D.cn[wc];
-        const int* lst = D.cl + wc * kCellCap;
-        for (int k = 0; k < n; ++k) {
-            int j = lst[k];
-            if (j == i) continue;
-            double3 rij = min_image(b, sub3(ri, d3(D.r[j])));
-            double rsq = sq3(rij);
-            if (rsq < P.rcut_max_sq) {
-                if (nn < kMaxNb) nb[nn++] = j; else over = true;
+    for (int k = 0; k < nc; ++k) {
+        const int i = own[k];
+        const double3 ri = d3(D.r[i]);
+        int nn = 0, nm = 0;
+        double rho = 0.0;
+        int* nb = D.nb + (size_t)i * kMaxNb;
+        int* nbm = D.nbm + (size_t)i * kMaxMag;
+        double* nbJ = D.nbJ + (size_t)i * kMaxMag;
+        for (int q = 0; q < npass; ++q) {
+            int j;
+            double4 rj;
+            if (q < kCandReg) {
+                // registers, indexed with a compile-time constant through the switch
+                j = jc[0]; rj = rc[0];
+                if (q == 1) { j = jc[1]; rj = rc[1]; }
+                if (q == 2) { j = jc[2]; rj = rc[2]; }
+                if (q == 3) { j = jc[3]; rj = rc[3]; }
+            } else {
+                int idx = q * 64 + lane;
+                j = idx < total ? s_cand[w][idx] : -1;
+                rj = j >= 0 ? D.r[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            bool ok = j >= 0 && j != i;
+            double rsq = 1e300;
+            if (ok) {
+                double3 rij = min_image(b, sub3(ri, d3(rj)));
+                rsq = sq3(rij);
+            }
+            const bool inmax = ok && rsq < P.rcut_max_sq;
+            const unsigned long long m1 = __ballot(inmax);
+            const int pos = nn + __popcll(m1 & lt);
+            if (inmax && pos < kMaxNb) nb[pos] = j;
+            nn += __popcll(m1);
+            const bool inmag = inmax && rsq < P.rcut_mag_sq;
+            const unsigned long long m2 = __ballot(inmag);
+            const int pos2 = nm + __popcll(m2 & lt);
+            if (inmax) {
                 double r = sqrt(rsq);
                 int n0;
                 double ratio;
                 interp_index(r, T.in_over_rmax, n0, ratio);
                 if (r < P.rcut_pot) rho += lerp_tab(T.rtab, kRStride, R_SF, n0, ratio);
-                if (rsq < P.rcut_mag_sq) {
-                    if (nm < kMaxMag) {
-                        nbm[nm] = j;
-                        nbJ[nm] = lerp_tab(T.rtab, kRStride, R_J, n0, ratio);
-                        ++nm;
-                    } else {
-                        over = true;
-                    }
+                if (inmag && pos2 < kMaxMag) {
+                    nbm[pos2] = j;
+                    nbJ[pos2] = lerp_tab(T.rtab, kRStride, R_J, n0, ratio);
                 }
             }
+            nm += __popcll(m2);
         }
+        rho = wave_sum(rho);
+        if (lane == 0) {
+            D.nbn[i] = nn < kMaxNb ? nn : kMaxNb;
+            D.nbmn[i] = nm < kMaxMag ? nm : kMaxMag;
+            if (write_rho) D.rho[i] = rho;
+        }
+        over |= nn > kMaxNb || nm > kMaxMag;
     }
-    D.nbn[i] = nn;
-    D.nbmn[i] = nm;
-    if (write_rho) D.rho[i] = rho;
-    if (over) atomicOr(D.err, kErrNbOverflow);
+    if (over && lane == 0) atomicOr(D.err, kErrNbOverflow);
 }
 
 // ---------------------------------------------------------------------------
@@ -302,69 +379,199 @@ __device__ __forceinline__ double3 landau_field(double3 s, double A, double B, d
     return make_double3((k + hc) * s.x, (k + hc) * s.y, (k + hc) * s.z);
 }
 
-__device__ __forceinline__ void spin_update(int a, int c, double dt, const Tables& T, const Phys& P,
-                                            const Dev& D, uint64_t pass) {
-    double3 H = make_double3(0.0, 0.0, 0.0);
-    double S = 0.0;
+// Exchange field H = sum J s_j and S = sum J |s_j| over the rcut_mag list,
+// gathered eight neighbours at a time with all loads of a batch in flight.
+__device__ __forceinline__ void exchange_field(int a, const Dev& D, double3& H, double& S) {
     const int nm = D.nbmn[a];
-    const int* nbm = D.nbm + (size_t)a * kMaxMag;
-    const double* nbJ = D.nbJ + (size_t)a * kMaxMag;
-    for (int k = 0; k < nm; ++k) {
-        double J = nbJ[k];
-        double4 sj = D.s[nbm[k]];
-        H.x += J * sj.x; H.y += J * sj.y; H.z += J * sj.z;
-        S += J * sj.w;
+    const int* ids = D.nbm + (size_t)a * kMaxMag;
+    const double* js = D.nbJ + (size_t)a * kMaxMag;
+    double hx = 0.0, hy = 0.0, hz = 0.0, ss = 0.0;
+#pragma unroll 1
+    for (int b0 = 0; b0 < nm; b0 += 8) {
+        const int4 i0 = *reinterpret_cast<const int4*>(ids + b0);
+        const int4 i1 = *reinterpret_cast<const int4*>(ids + b0 + 4);
+        const int id[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+        double J[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double2 v = *reinterpret_cast<const double2*>(js + b0 + 2 * q);
+            J[2 * q] = v.x; J[2 * q + 1] = v.y;
+        }
+        double4 sj[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sj[q] = D.s[b0 + q < nm ? id[q] : a];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double Jq = b0 + q < nm ? J[q] : 0.0;
+            hx += Jq * sj[q].x; hy += Jq * sj[q].y; hz += Jq * sj[q].z;
+            ss += Jq * sj[q].w;
+        }
     }
-    double3 s = d3(D.s[a]);
-    double3 L = make_double3(0.0, 0.0, 0.0);
-    if (!P.spinlang) {
-        s = spin_rotation(H, s, dt);
-    } else {
-        const double dth = dt / 2.0, dtq = dt / 4.0, g = P.gamma_S;
-        s = spin_rotation(H, s, dth);
-        int n0;
-        double ratio;
-        interp_index(D.rho[a], T.in_over_rhomax, n0, ratio);
-        double A = lerp_tab(T.rhotab, kRhoStride, H_A, n0, ratio);
-        double B = lerp_tab(T.rhotab, kRhoStride, H_B, n0, ratio);
-        double C = lerp_tab(T.rhotab, kRhoStride, H_C, n0, ratio);
-        double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
-        // RK2
-        L = landau_field(s, A, B, C, Dd, S);
-        double3 st = add3(s, mul3(g * dtq, add3(H, L)));
-        L = landau_field(st, A, B, C, Dd, S);
-        s = add3(s, mul3(g * dth, add3(H, L)));
-        // noise
-        double Tk = P.eltemp ? D.Te[c] : P.temperature;
-        double rs = sqrt(2.0 * Tk * g / dt);
-        double3 z = normal3(P.seed, P.replica, a, pass);
-        s = add3(s, mul3(dt, mul3(rs, z)));
-        // RK2
-        L = landau_field(s, A, B, C, Dd, S);
-        st = add3(s, mul3(g * dtq, add3(H, L)));
-        L = landau_field(st, A, B, C, Dd, S);
-        s = add3(s, mul3(g * dth, add3(H, L)));
-        s = spin_rotation(H, s, dth);
-        D.sjs[a] = S;
-    }
-    D.s[a] = make_double4(s.x, s.y, s.z, sqrt(sq3(s)));
-    D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
-    D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
+    H = make_double3(hx, hy, hz);
+    S = ss;
 }
 
-// One dependency level of a core_ds sweep: every listed cell walks its list
-// head to tail (forward) or tail to head (backward).
-__global__ __launch_bounds__(64) void k_spin_phase(const int* cells, int ncell, int backward, double dt,
-                                                   Tables T, Phys P, Dev D, uint64_t pass) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncell) return;
-    int c = cells[t];
-    int n = D.cn[c];
+// One dependency level of a core_ds sweep: each listed cell walks its list
+// head to tail (forward) or tail to head (backward), one thread per cell
+// (calculate_spin_CPU.cpp:33-235, core_ds_CPU.cpp:35-83).
+//
+// The per-atom work is software pipelined so that the neighbour gathers run
+// under the arithmetic of the previous atom: during the update of atom q the
+// thread already holds atom q+1's neighbour ids and J values and has issued
+// the loads of those neighbours' spins; atom q+2's ids are in flight. The
+// only spin such a load can miss is atom q's own new value (written after
+// the load was issued), which is patched in from registers, so every atom
+// sees exactly the spins the sequential sweep shows it.
+
+constexpr int kPipe = 16;   // rcut_mag neighbours held in registers (more: loaded directly)
+
+__device__ __forceinline__ void load_ids(const Dev& D, int a, int (&id)[kPipe]) {
+    const int4* p = reinterpret_cast<const int4*>(D.nbm + (size_t)a * kMaxMag);
+#pragma unroll
+    for (int q = 0; q < kPipe / 4; ++q) {
+        int4 v = p[q];
+        id[4 * q] = v.x; id[4 * q + 1] = v.y; id[4 * q + 2] = v.z; id[4 * q + 3] = v.w;
+    }
+}
+
+__device__ __forceinline__ void load_js(const Dev& D, int a, double (&J)[kPipe]) {
+    const double2* p = reinterpret_cast<const double2*>(D.nbJ + (size_t)a * kMaxMag);
+#pragma unroll
+    for (int q = 0; q < kPipe / 2; ++q) {
+        double2 v = p[q];
+        J[2 * q] = v.x; J[2 * q + 1] = v.y;
+    }
+}
+
+// rotation of s about Heff by |Heff| dt / hbar (core_ds_CPU.cpp:90-127); the
+// axis and angle are computed once for both half-rotations of a step
+struct Rot {
+    double wx, wy, wz, A, B;
+};
+
+__device__ __forceinline__ Rot make_rot(double3 H, double dt) {
+    double3 w = make_double3(H.x / -kHbar, H.y / -kHbar, H.z / -kHbar);
+    double w0 = sqrt(sq3(w));
+    Rot r;
+    if (w0 > 0.0) { r.wx = w.x / w0; r.wy = w.y / w0; r.wz = w.z / w0; }
+    else { r.wx = 0.0; r.wy = 0.0; r.wz = 0.0; }
+    double sn, cs;
+    sincos(w0 * dt, &sn, &cs);
+    r.A = sn;
+    r.B = 1.0 - cs;
+    return r;
+}
+
+__device__ __forceinline__ double3 apply_rot(const Rot& r, double3 s) {
+    double o12 = r.wx * r.wy, o23 = r.wy * r.wz, o13 = r.wx * r.wz;
+    double o1s = r.wx * r.wx, o2s = r.wy * r.wy, o3s = r.wz * r.wz;
+    double A = r.A, B = r.B;
+    double3 t;
+    t.x = s.x + (s.x * B * (-o2s - o3s) + s.y * (B * o12 - A * r.wz) + s.z * (A * r.wy + B * o13));
+    t.y = s.y + (s.y * B * (-o1s - o3s) + s.z * (B * o23 - A * r.wx) + s.x * (A * r.wz + B * o12));
+    t.z = s.z + (s.z * B * (-o1s - o2s) + s.x * (B * o13 - A * r.wy) + s.y * (A * r.wx + B * o23));
+    return t;
+}
+
+__global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell, int backward, double dt,
+                                                    Tables T, Phys P, Dev D, uint64_t pass) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ncell) return;
+    const int c = cells[g];
+    const int n = D.cn[c];
+    if (n == 0) return;
     const int* lst = D.cl + c * kCellCap;
-    if (!backward) {
-        for (int k = 0; k < n; ++k) spin_update(lst[k], c, dt, T, P, D, pass);
-    } else {
-        for (int k = n - 1; k >= 0; --k) spin_update(lst[k], c, dt, T, P, D, pass);
+    const double Tk = P.spinlang ? (P.eltemp ? D.Te[c] : P.temperature) : 0.0;
+    const double dth = dt / 2.0, dtq = dt / 4.0, gS = P.gamma_S;
+    const double rs = P.spinlang ? sqrt(2.0 * Tk * gS / dt) : 0.0;
+    auto atom_at = [&](int q) { return lst[backward ? n - 1 - q : q]; };
+
+    // atom q: ids/J/spins in registers; atom q+1: ids in registers
+    int a_c = atom_at(0), nm_c = D.nbmn[a_c];
+    int id_c[kPipe], id_n[kPipe];
+    double J_c[kPipe];
+    double4 sj[kPipe];
+    load_ids(D, a_c, id_c);
+    load_js(D, a_c, J_c);
+#pragma unroll
+    for (int k = 0; k < kPipe; ++k) sj[k] = D.s[k < nm_c ? id_c[k] : a_c];
+    int a_n = a_c, nm_n = nm_c;
+    if (n > 1) {
+        a_n = atom_at(1);
+        nm_n = D.nbmn[a_n];
+        load_ids(D, a_n, id_n);
+    }
+    int prev_a = -1;
+    double4 prev_s = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int q = 0; q < n; ++q) {
+        const int a = a_c;
+        // field of atom q from the gathered spins, with the previous atom's
+        // new spin patched in
+        double hx = 0.0, hy = 0.0, hz = 0.0, S = 0.0;
+#pragma unroll
+        for (int k = 0; k < kPipe; ++k) {
+            double4 v = id_c[k] == prev_a ? prev_s : sj[k];
+            double Jk = k < nm_c ? J_c[k] : 0.0;
+            hx += Jk * v.x; hy += Jk * v.y; hz += Jk * v.z;
+            S += Jk * v.w;
+        }
+        for (int k = kPipe; k < nm_c; ++k) {
+            double Jk = D.nbJ[(size_t)a * kMaxMag + k];
+            double4 v = D.s[D.nbm[(size_t)a * kMaxMag + k]];
+            hx += Jk * v.x; hy += Jk * v.y; hz += Jk * v.z;
+            S += Jk * v.w;
+        }
+        const double3 H = make_double3(hx, hy, hz);
+        // the inputs of this atom that do not depend on other atoms
+        double3 s = d3(D.s[a]);
+        const double rho = D.rho[a];
+        // pipeline: issue atom q+1's spin and J loads and atom q+2's ids
+        if (q + 1 < n) {
+#pragma unroll
+            for (int k = 0; k < kPipe; ++k) {
+                id_c[k] = id_n[k];
+                sj[k] = D.s[k < nm_n ? id_n[k] : a_n];
+            }
+            load_js(D, a_n, J_c);
+            a_c = a_n;
+            nm_c = nm_n;
+            if (q + 2 < n) {
+                a_n = atom_at(q + 2);
+                nm_n = D.nbmn[a_n];
+                load_ids(D, a_n, id_n);
+            }
+        }
+        double3 L = make_double3(0.0, 0.0, 0.0);
+        if (!P.spinlang) {
+            s = apply_rot(make_rot(H, dt), s);
+        } else {
+            double3 z = normal3(P.seed, P.replica, a, pass);
+            int n0;
+            double ratio;
+            interp_index(rho, T.in_over_rhomax, n0, ratio);
+            double A = lerp_tab(T.rhotab, kRhoStride, H_A, n0, ratio);
+            double B = lerp_tab(T.rhotab, kRhoStride, H_B, n0, ratio);
+            double C = lerp_tab(T.rhotab, kRhoStride, H_C, n0, ratio);
+            double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
+            const Rot R = make_rot(H, dth);
+            s = apply_rot(R, s);
+            L = landau_field(s, A, B, C, Dd, S);
+            double3 st = add3(s, mul3(gS * dtq, add3(H, L)));
+            L = landau_field(st, A, B, C, Dd, S);
+            s = add3(s, mul3(gS * dth, add3(H, L)));
+            s = add3(s, mul3(dt, mul3(rs, z)));
+            L = landau_field(s, A, B, C, Dd, S);
+            st = add3(s, mul3(gS * dtq, add3(H, L)));
+            L = landau_field(st, A, B, C, Dd, S);
+            s = add3(s, mul3(gS * dth, add3(H, L)));
+            s = apply_rot(R, s);
+            D.sjs[a] = S;
+        }
+        prev_s = make_double4(s.x, s.y, s.z, sqrt(sq3(s)));
+        prev_a = a;
+        D.s[a] = prev_s;
+        D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
+        D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
     }
 }
 
@@ -372,70 +579,78 @@ __global__ __launch_bounds__(64) void k_spin_phase(const int* cells, int ncell, 
 // momenta (core_dp_CPU.cpp:51-312). Every part of the Langevin update is local
 // to one link cell, so a thread runs C1, B, A, B, C2 for its cell in order.
 
-__device__ __forceinline__ void dp_colmot(const int* lst, int n, double factor, bool backward, Dev D) {
-    for (int q = 0; q < n; ++q) {
-        int a = lst[backward ? n - 1 - q : q];
-        double3 sp = make_double3(0.0, 0.0, 0.0);
-        for (int w = 0; w < n; ++w) sp = add3(sp, d3(D.p[lst[w]]));
-        double3 pa = d3(D.p[a]);
-        pa = add3(pa, mul3(factor, sub3(sp, pa)));
-        D.p[a] = make_double4(pa.x, pa.y, pa.z, 0.0);
-    }
+__device__ __forceinline__ double seg_sum32(double v) {
+#pragma unroll
+    for (int m = 16; m > 0; m >>= 1) v += __shfl_xor(v, m, 32);
+    return v;
 }
 
+__device__ __forceinline__ double3 seg_sum32(double3 v) {
+    return make_double3(seg_sum32(v.x), seg_sum32(v.y), seg_sum32(v.z));
+}
+
+// Local collective motion step: walking the list (head to tail, or tail to
+// head), each atom gets p += g dt/n (sum_p - p) with sum_p over the cell's
+// current momenta (core_dp_CPU.cpp:211-264). Lane l holds list entry l.
+// The cell sum is carried along: after atom k changes by d, the sum changes
+// by d (the reference re-adds the list each time; same value up to rounding).
+__device__ __forceinline__ double3 dp_colmot(double3 p, int l, int n, double factor, bool backward) {
+    double3 sp = seg_sum32(p);
+    const int seg = threadIdx.x & 32;
+    for (int q = 0; q < n; ++q) {
+        const int k = backward ? n - 1 - q : q;
+        double3 d = make_double3(0.0, 0.0, 0.0);
+        if (l == k) {
+            double3 pn = add3(p, mul3(factor, sub3(sp, p)));
+            d = sub3(pn, p);
+            p = pn;
+        }
+        const int src = seg + k;
+        d.x = __shfl(d.x, src, 64);
+        d.y = __shfl(d.y, src, 64);
+        d.z = __shfl(d.z, src, 64);
+        sp = add3(sp, d);
+    }
+    return p;
+}
+
+// One half-wave (32 lanes) per link cell, one atom per lane.
 __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, uint64_t pass,
                                                   double* partials) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (c < D.C) {
-        const int n = D.cn[c];
-        const int* lst = D.cl + c * kCellCap;
+    const int l = threadIdx.x & 31;
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const bool valid = c < D.C;
+    const int n = valid ? D.cn[c] : 0;
+    const bool has = l < n;
+    const int a = has ? D.cl[c * kCellCap + l] : 0;
+    double3 p = has ? d3(D.p[a]) : make_double3(0.0, 0.0, 0.0);
+    if (valid) {
         const double gm = P.gamma_L / P.atmass;
         if (P.lattlang) {
             const double h = dt / 2.0;
-            if (P.localcolmot && n > 0) dp_colmot(lst, n, gm * h / n, false, D);
+            if (P.localcolmot && n > 0) p = dp_colmot(p, l, n, gm * h / n, false);
             const double eb = P.localcolmot ? exp(-gm * h * (1.0 - 1.0 / n)) : exp(-gm * h);
-            for (int q = 0; q < n; ++q) {
-                int a = lst[q];
-                double4 p = D.p[a];
-                D.p[a] = make_double4(eb * p.x, eb * p.y, eb * p.z, 0.0);
-            }
+            p = mul3(eb, p);
             const double Tk = P.eltemp ? D.Te[c] : P.temperature;
             const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
-            double3 fsum = make_double3(0.0, 0.0, 0.0);
-            for (int q = 0; q < n; ++q) {
-                int a = lst[q];
-                double3 ff = mul3(len, normal3(P.seed, P.replica, a, pass));
-                double3 pa = add3(d3(D.p[a]), mul3(dt, add3(d3(D.f[a]), ff)));
-                D.p[a] = make_double4(pa.x, pa.y, pa.z, 0.0);
-                fsum = add3(fsum, ff);
+            double3 ff = make_double3(0.0, 0.0, 0.0);
+            if (has) {
+                ff = mul3(len, normal3(P.seed, P.replica, a, pass));
+                p = add3(p, mul3(dt, add3(d3(D.f[a]), ff)));
             }
             if (P.localcolmot && n > 0) {
-                double3 ave = make_double3(fsum.x / n, fsum.y / n, fsum.z / n);
-                for (int q = 0; q < n; ++q) {
-                    int a = lst[q];
-                    double3 pa = sub3(d3(D.p[a]), mul3(dt, ave));
-                    D.p[a] = make_double4(pa.x, pa.y, pa.z, 0.0);
-                }
+                double3 fs = seg_sum32(ff);
+                double3 ave = make_double3(fs.x / n, fs.y / n, fs.z / n);
+                if (has) p = sub3(p, mul3(dt, ave));
             }
-            for (int q = 0; q < n; ++q) {
-                int a = lst[q];
-                double4 p = D.p[a];
-                D.p[a] = make_double4(eb * p.x, eb * p.y, eb * p.z, 0.0);
-            }
-            if (P.localcolmot && n > 0) dp_colmot(lst, n, gm * h / n, true, D);
-        } else {
-            for (int q = 0; q < n; ++q) {
-                int a = lst[q];
-                double3 pa = add3(d3(D.p[a]), mul3(dt, d3(D.f[a])));
-                D.p[a] = make_double4(pa.x, pa.y, pa.z, 0.0);
-            }
+            p = mul3(eb, p);
+            if (P.localcolmot && n > 0) p = dp_colmot(p, l, n, gm * h / n, true);
+        } else if (has) {
+            p = add3(p, mul3(dt, d3(D.f[a])));
         }
-        for (int q = 0; q < n; ++q) {
-            double4 p = D.p[lst[q]];
-            acc[0] += p.x; acc[1] += p.y; acc[2] += p.z;
-        }
+        if (has) D.p[a] = make_double4(p.x, p.y, p.z, 0.0);
     }
+    double acc[3] = {has ? p.x : 0.0, has ? p.y : 0.0, has ? p.z : 0.0};
     block_partials<3>(acc, partials);
 }
 
@@ -479,32 +694,26 @@ __global__ __launch_bounds__(256) void k_dr(double dt, Box b, Phys P, Dev D) {
 // ---------------------------------------------------------------------------
 // electron subsystem
 
-// calculate_temperature_CPU.cpp:35-148, one thread per link cell
-__global__ __launch_bounds__(128) void k_temperature(Tables T, Phys P, Dev D) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+// calculate_temperature_CPU.cpp:35-148: one half-wave per link cell, one atom
+// per lane; the cell sums are lane reductions.
+__global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
+    const int l = threadIdx.x & 31;
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     if (c >= D.C) return;
     const int n = D.cn[c];
-    const int* lst = D.cl + c * kCellCap;
-    double sum_ke = 0.0, rup = 0.0, rdn = 0.0, lup = 0.0, ldn = 0.0;
-    double3 ap = make_double3(0.0, 0.0, 0.0);
-    for (int q = 0; q < n; ++q) {
-        int a = lst[q];
-        sum_ke += D.ke[a];
-        if (P.localcolmot) ap = add3(ap, d3(D.p[a]));
-        double3 H = make_double3(0.0, 0.0, 0.0);
-        double S = 0.0;
-        const int nm = D.nbmn[a];
-        const int* nbm = D.nbm + (size_t)a * kMaxMag;
-        const double* nbJ = D.nbJ + (size_t)a * kMaxMag;
-        for (int k = 0; k < nm; ++k) {
-            double J = nbJ[k];
-            double4 sj = D.s[nbm[k]];
-            H.x += J * sj.x; H.y += J * sj.y; H.z += J * sj.z;
-            S += J * sj.w;
-        }
+    const bool has = l < n;
+    double ke = 0.0, rup = 0.0, rdn = 0.0, lup = 0.0, ldn = 0.0;
+    double3 pa = make_double3(0.0, 0.0, 0.0);
+    if (has) {
+        const int a = D.cl[c * kCellCap + l];
+        ke = D.ke[a];
+        pa = d3(D.p[a]);
+        double3 H;
+        double S;
+        exchange_field(a, D, H, S);
         double3 s = d3(D.s[a]);
-        rup += sq3(cross3(s, H));
-        rdn += dot3(s, H);
+        rup = sq3(cross3(s, H));
+        rdn = dot3(s, H);
         int n0;
         double ratio;
         interp_index(D.rho[a], T.in_over_rhomax, n0, ratio);
@@ -517,14 +726,20 @@ __global__ __launch_bounds__(128) void k_temperature(Tables T, Phys P, Dev D) {
         double k = -(2.0 * A + 4.0 * B * s2 + 6.0 * C * s4 + 8.0 * Dd * s4 * s2);
         double hc = -S / s0;
         double3 L = make_double3((k + hc) * s.x, (k + hc) * s.y, (k + hc) * s.z);
-        lup += sq3(add3(H, L));
-        ldn += 6.0 * A + 20.0 * B * s2 + 42.0 * C * s4 + 72.0 * Dd * s4 * s2;
-        ldn += 2.0 * S / s0;
+        lup = sq3(add3(H, L));
+        ldn = 6.0 * A + 20.0 * B * s2 + 42.0 * C * s4 + 72.0 * Dd * s4 * s2 + 2.0 * S / s0;
         D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
         D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
         D.sjs[a] = S;
         D.s[a].w = s0;
     }
+    double sum_ke = seg_sum32(ke);
+    double3 ap = seg_sum32(pa);
+    rup = seg_sum32(rup);
+    rdn = seg_sum32(rdn);
+    lup = seg_sum32(lup);
+    ldn = seg_sum32(ldn);
+    if (l != 0) return;
     const double apc = (double)D.N / (double)D.C;
     const double gm = P.gamma_L / P.atmass;
     if (P.localcolmot) {
@@ -584,27 +799,68 @@ __global__ __launch_bounds__(256) void k_dte_ab(double h, Phys P, Dev D, double*
     block_partials<1>(acc, partials);
 }
 
-// one dependency level of the C diffusion sweep (core_dTe_CPU.cpp:155-197)
-__global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, Box b, double pref,
+// one dependency level of the C diffusion sweep (core_dTe_CPU.cpp:155-197).
+// The reference recomputes E(Te) of the cell first; the stored Ee of the
+// cell's last update is that value up to one rounding.
+__global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref,
                                                    double dx2, double dy2, double dz2, Dev D) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncell) return;
-    int c = cells[t];
+    const int c = cells[t];
+    const int* f = D.fn6 + 6 * c;
     const double* Te = D.Te;
-    double tc = Te[c];
-    double ee = te_to_ee(tc);
-    ee += pref * ((Te[neigh_cell(b, c, 0)] + Te[neigh_cell(b, c, 13)] - 2.0 * tc) / dx2 +
-                  (Te[neigh_cell(b, c, 2)] + Te[neigh_cell(b, c, 14)] - 2.0 * tc) / dy2 +
-                  (Te[neigh_cell(b, c, 12)] + Te[neigh_cell(b, c, 15)] - 2.0 * tc) / dz2);
+    const double tc = Te[c];
+    const double t0 = Te[f[0]], t1 = Te[f[1]], t2 = Te[f[2]], t3 = Te[f[3]], t4 = Te[f[4]],
+                 t5 = Te[f[5]];
+    double ee = D.Ee[c];
+    ee += pref * ((t0 + t1 - 2.0 * tc) / dx2 + (t2 + t3 - 2.0 * tc) / dy2 + (t4 + t5 - 2.0 * tc) / dz2);
     D.Ee[c] = ee;
     D.Te[c] = ee_to_te(ee);
 }
 
+// end of C: total electron energy from the cells' last updates
 __global__ __launch_bounds__(256) void k_dte_end(Dev D, double* partials) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {c < D.C ? D.Ee[c] : 0.0};
+    block_partials<1>(acc, partials);
+}
+
+// A(h) when E(Te) is already known (the previous A left it in Ee)
+__device__ __forceinline__ void dte_A_known(int c, double h, const Phys& P, const Dev& D, double ee) {
+    double dE = 0.0;
+    if (P.lattlang) dE += D.Gel[c] * D.Tl[c];
+    if (P.spinlang) dE += D.Ges[c] * D.TsL[c];
+    ee += h * dE;
+    D.Ee[c] = ee;
+    D.Te[c] = ee_to_te(ee);
+}
+
+// starting and final totals of a sub-step from n partial pairs (xp[0..n):
+// start, xp[n..2n): end), summed in a fixed order by every thread
+__device__ __forceinline__ double dte_factor(const double* xp, int n) {
+    double st = 0.0, en = 0.0;
+    for (int k = 0; k < n; ++k) {
+        st += xp[k];
+        en += xp[n + k];
+    }
+    return st / en;
+}
+
+// end of one sub-step and start of the next: renormalise (core_dTe_CPU.cpp:199-216),
+// B(h), A(h), then the next sub-step's A(h), B(h) and E(Te) as its C starts
+__global__ __launch_bounds__(256) void k_dte_finish_ab(double h, Phys P, Dev D, const double* xp, int nxp,
+                                                       double* partials) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (c < D.C) {
-        double ee = te_to_ee(D.Te[c]);
+        double f = dte_factor(xp, nxp);
+        double ee = D.Ee[c] * f;
+        D.Te[c] = ee_to_te(ee);
+        dte_B(c, h, P, D);
+        dte_A(c, h, P, D);
+        dte_A_known(c, h, P, D, D.Ee[c]);
+        dte_B(c, h, P, D);
+        ee = te_to_ee(D.Te[c]);
         D.Ee[c] = ee;
         acc[0] = ee;
     }
@@ -612,16 +868,17 @@ __global__ __launch_bounds__(256) void k_dte_end(Dev D, double* partials) {
 }
 
 // renormalise to the starting total (core_dTe_CPU.cpp:199-216), then B(h), A(h)
-__global__ __launch_bounds__(256) void k_dte_finish(double h, Phys P, Dev D, const double* se) {
+__global__ __launch_bounds__(256) void k_dte_finish(double h, Phys P, Dev D, const double* xp, int nxp) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.C) return;
-    double f = se[0] / se[1];
+    double f = dte_factor(xp, nxp);
     double ee = D.Ee[c] * f;
     D.Ee[c] = ee;
     D.Te[c] = ee_to_te(ee);
     dte_B(c, h, P, D);
     dte_A(c, h, P, D);
 }
+
 
 // ---------------------------------------------------------------------------
 // observables (check_energy_CPU.cpp, check_temperature_CPU.cpp, check_spin_CPU.cpp)
@@ -642,17 +899,9 @@ __global__ __launch_bounds__(256) void k_obs_atoms(Tables T, Phys P, Dev D, doub
         acc[11] = D.vir[a];
         if (scan_spin) {
             // check_temperature_CPU.cpp:64-77,97-128 (no eltemp)
-            double3 H = make_double3(0.0, 0.0, 0.0);
-            double S = 0.0;
-            const int nm = D.nbmn[a];
-            const int* nbm = D.nbm + (size_t)a * kMaxMag;
-            const double* nbJ = D.nbJ + (size_t)a * kMaxMag;
-            for (int k = 0; k < nm; ++k) {
-                double J = nbJ[k];
-                double4 sj = D.s[nbm[k]];
-                H.x += J * sj.x; H.y += J * sj.y; H.z += J * sj.z;
-                S += J * sj.w;
-            }
+            double3 H;
+            double S;
+            exchange_field(a, D, H, S);
             acc[7] = sq3(cross3(s, H));
             acc[8] = dot3(s, H);
             int n0;

@@ -11,6 +11,7 @@ import pytest
 
 from helpers import TOL, assert_close, consistent_state, fe_params, relerr
 from oracle import orcpy
+from spilady_amd import constants as K
 
 pytestmark = pytest.mark.gpu
 
@@ -121,4 +122,34 @@ def test_cell_lists_after_movers():
     c2, l2 = eng.cell_lists(32)
     np.testing.assert_array_equal(c1, c2)
     np.testing.assert_array_equal(l1, l2)
+    eng.close(); orc.close()
+
+
+@pytest.mark.parametrize("nu", [16, 24, 31])
+def test_core_dte_hot_spot(nu):
+    """core_dTe on a non-uniform electron temperature (a hot sphere over
+    cell-to-cell noise), three calls of 50 sub-steps each. 16^3 and 31^3 unit
+    cells give odd link-cell counts per side (11, 21: 18 sweep levels), 24^3
+    an even one (16: 4 levels)."""
+    p, eng, orc = _pair(nu, 600.0, 5, thermostats=True, eltemp=True, localcolmot=True,
+                        renormalize=False)
+    for o in (orc, eng):
+        o.links()
+        o.calculate_temperature()
+    nx, ny, nz = orc.ncell3
+    c = np.arange(nx * ny * nz)
+    x, y, z = c % nx, (c // nx) % ny, c // (nx * ny)
+    d2 = (x - nx / 2) ** 2 + (y - ny / 2) ** 2 + (z - nz / 2) ** 2
+    te_k = 600.0 + 2400.0 * np.exp(-d2 / (0.1 * nx * nx)) + np.random.default_rng(nu).uniform(0, 50, c.size)
+    te = te_k * K.BOLTZ   # cells hold Te in eV
+    for o in (orc, eng):
+        o.set_cells_te(te)
+    dt = p.step
+    for k in range(3):
+        orc.core_dTe(dt / 2)
+        eng.core_dTe(dt / 2)
+        a, b = orc.cells()["te"], eng.cells()["te"]
+        assert_close(a, b, f"dTe call {k}", TOL)
+    # diffusion moved heat out of the hot spot, and the total is kept
+    assert b.max() < te.max()
     eng.close(); orc.close()
