@@ -35,27 +35,55 @@ METRIC = "atom-steps/s at 2M-atom bcc Fe SLD (FP64); batched-vs-serial replica s
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic FP64 work per unit, counted from the reference formulas with
-# transcendentals, square roots and divisions as one operation each:
-# W_F = 192 per ordered pair (force), 26 pairs per atom in bcc Fe within
-# rcut_pot, plus a 70-operation one-body epilogue; W_H = 31 and W_J = 33 per
-# magnetic pair (14 per atom) and U = 300 for the Langevin SLDHL spin update
-# (57 without thermostat); rho + neighbour build W_rho = 26 per pair.
+# Algorithmic work per unit, counted from the reference formulas with
+# transcendentals, square roots and divisions as one operation each, and
+# compulsory HBM bytes per unit (state read and written once, gathered
+# neighbour data assumed on chip). bcc Fe: 26 neighbours within rcut_pot and
+# 14 within rcut_mag per atom.
 PAIRS_POT = 26
 PAIRS_MAG = 14
-FLOP = {
-    "k_force": PAIRS_POT * 192 + 70,                 # per atom
-    "k_spin_phase": PAIRS_MAG * (31 + 33) + 300,     # per atom visit (4 visits per step)
-    "k_nbuild": PAIRS_POT * 26 + PAIRS_MAG * 7,      # per atom
+KERNELS = {
+    # one cell update of the diffusion sweep: 3 x (2 add, 1 mul, 1 sub, 1 div)
+    # + 3 add + 1 mul, then T(E) = acosh(exp(bE/3))/b (4); reads Te, Ee of the
+    # cell and writes both (the six neighbour Te come from cache)
+    "k_dte_sweep": {"unit": "cell update", "flop": 23, "bytes": 32},
+    # one atom visit of the spin sweep: field W_H + W_J = 64 per magnetic pair,
+    # 300 for the Langevin SLDHL update; reads s, rho, the 14 neighbour ids
+    # and J values, writes s, H_eff, H_L and sum J|s_j|
+    "k_spin_phase": {"unit": "atom visit", "flop": PAIRS_MAG * (31 + 33) + 300,
+                     "bytes": 32 + 8 + 4 + PAIRS_MAG * (4 + 8) + 3 * 32 + 8},
+    # one atom of the force pass: W_F = 192 per pair + 70; reads r, s, rho,
+    # F'(rho) and the neighbour ids, writes f, pe, me, me0, vir, stress[6]
+    "k_force": {"unit": "atom", "flop": PAIRS_POT * 192 + 70,
+                "bytes": 3 * 32 + 2 * 8 + 4 + PAIRS_POT * 4 + 32 + 4 * 8 + 6 * 8},
 }
-# Compulsory HBM bytes per unit for the same kernels (FP64 state read and
-# written once; gathered neighbour data assumed on chip).
-BYTES = {
-    "k_force": 32 * 3 + 8 * 2 + 4 * PAIRS_POT + 4 + 32 + 8 * 4 + 8 * 6,
-    "k_spin_phase": 32 + 8 + 4 * PAIRS_MAG + 8 * PAIRS_MAG + 4 + 32 * 3 + 8,
-    "k_nbuild": 32 + 4 * PAIRS_POT + 12 * PAIRS_MAG + 8 + 8,
-}
-VISITS_PER_STEP = {"k_force": 1, "k_spin_phase": 4, "k_nbuild": 1}
+# whole step: FLOP and bytes per atom-step of the full SLDHL step with
+# eltemp (electron cell state spread over 6.09 atoms per cell)
+STEP_FLOP = 14.6e3
+STEP_BYTES = 381.0
+NLOOP_DTE = 50
+
+
+def traffic_of(kname: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the same counter reads for k_dr, a
+    streaming kernel of known traffic; read from the committed summary when it
+    covers this kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        return t["kernels"][kname]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def units_per_step(kname: str, natom: int, ncell: int) -> int:
+    if kname == "k_dte_sweep":
+        return 2 * ncell * 2 * NLOOP_DTE   # two core_dTe calls, forward + backward sweep
+    if kname == "k_spin_phase":
+        return 4 * natom                   # two core_ds calls, forward + backward sweep
+    return natom
 
 
 def build_workload(n_unit: int):
@@ -109,7 +137,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-unit", type=int, default=100, help="cubic cells per side (100 = 2M atoms)")
-    ap.add_argument("--timed-kernel", default="k_spin_phase")
+    ap.add_argument("--kernels", default=",".join(KERNELS),
+                    help="kernels timed with HIP events in extra steps after the timed region")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -124,31 +153,60 @@ def main() -> int:
     eng = Engine(p, at, compute_initial_forces=True)
     dev, ncu = device_info()
     eng.step(0, args.warmup)
-    eng.time_kernel(args.timed_kernel)
     t0 = time.perf_counter()
     eng.step(args.warmup, args.steps)
     t1 = time.perf_counter()
     wall = t1 - t0
     dev_ms = eng.last_step_ms()
-    launches, kms = eng.kernel_timing()
-    eng.time_kernel(None)
+    # per-kernel durations from HIP events on the engine's stream, one extra
+    # step per kernel (outside the timed region)
+    kern = {}
+    s0 = args.warmup + args.steps
+    for k, kname in enumerate(x for x in args.kernels.split(",") if x):
+        eng.time_kernel(kname)
+        eng.step(s0 + k, 1)
+        launches, kms = eng.kernel_timing()
+        eng.time_kernel(None)
+        step_ms = eng.last_step_ms()
+        if launches == 0 or kname not in KERNELS:
+            continue
+        spec = KERNELS[kname]
+        units = units_per_step(kname, n, eng.ncells)
+        per_launch = units / launches
+        avg_s = kms / launches * 1e-3
+        flop_rate = spec["flop"] * per_launch / avg_s / 1e12
+        byte_rate = spec["bytes"] * per_launch / avg_s / 1e9
+        intensity = spec["flop"] / spec["bytes"]
+        hbm_bound = intensity < PEAK_FP64_TFLOPS * 1e3 / PEAK_HBM_GBS
+        kern[kname] = {
+            "bound": "hbm" if hbm_bound else "fp64",
+            "achieved": byte_rate if hbm_bound else flop_rate,
+            "peak": PEAK_HBM_GBS if hbm_bound else PEAK_FP64_TFLOPS,
+            "unit": "GB/s" if hbm_bound else "TFLOP/s",
+            "frac": byte_rate / PEAK_HBM_GBS if hbm_bound else flop_rate / PEAK_FP64_TFLOPS,
+            "launches_per_step": launches, "avg_launch_us": kms / launches * 1e3,
+            "units_per_launch": per_launch, "unit_of_work": spec["unit"],
+            "algorithmic_bytes_per_launch": spec["bytes"] * per_launch,
+            "algorithmic_flop_per_launch": spec["flop"] * per_launch,
+            "share_of_step": kms / step_ms if step_ms > 0 else None,
+        }
     st = eng.download() if not args.no_cpu_baseline else None
     eng.close()
 
     value = n * args.steps / wall
-    kname = args.timed_kernel
     roof = None
-    if kname in FLOP and launches > 0:
-        units = n * VISITS_PER_STEP[kname] * args.steps
-        flops = FLOP[kname] * units
-        tflops = flops / (kms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": tflops / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": kname, "launches": launches,
-                "avg_launch_ms": kms / launches,
-                "algorithmic_flop_per_launch": flops / launches,
-                "hbm_gbs_algorithmic": BYTES[kname] * units / (kms * 1e-3) / 1e9,
-                "kernel_share_of_step": kms / dev_ms if dev_ms > 0 else None}
+    if kern:
+        dom = max(kern, key=lambda k: kern[k]["share_of_step"] or 0.0)
+        r = kern[dom]
+        roof = {"bound": r["bound"], "achieved": r["achieved"], "peak": r["peak"], "unit": r["unit"],
+                "frac": r["frac"], "traffic": traffic_of(dom), "kernel": dom,
+                "avg_launch_us": r["avg_launch_us"], "launches_per_step": r["launches_per_step"],
+                "share_of_step": r["share_of_step"]}
+    step_roof = {"flop_per_atom_step": STEP_FLOP, "bytes_per_atom_step": STEP_BYTES,
+                 "fp64_tflops": value * STEP_FLOP / 1e12,
+                 "fp64_frac": value * STEP_FLOP / 1e12 / PEAK_FP64_TFLOPS,
+                 "hbm_gbs": value * STEP_BYTES / 1e9,
+                 "hbm_frac": value * STEP_BYTES / 1e9 / PEAK_HBM_GBS}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(v, st, at.box, n, args.cpu_steps)
@@ -171,6 +229,8 @@ def main() -> int:
                                "dt 1 fs, observables every step",
                    "natom": n, "device": dev, "cu": ncu},
         "roofline": roof,
+        "kernels": kern,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
     if cpu:

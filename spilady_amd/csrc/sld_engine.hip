@@ -64,6 +64,8 @@ struct sld_ctx {
     std::string timed_name;
     std::vector<hipEvent_t> ev_pool;
     int ev_used = 0;
+    std::vector<int> ev_launches;  // launches covered by each event pair
+    bool ev_span = false;          // an event pair spans a run of launches
     int timed_launches = 0;
     double timed_ms = 0.0;
     std::string err;
@@ -103,7 +105,7 @@ inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
 
 // Event pair around a launch of the timed kernel (see sld_time_kernel).
 hipEvent_t* timing_slot(sld_ctx* c, const char* name) {
-    if (c->timed_name.empty() || c->timed_name != name) return nullptr;
+    if (c->ev_span || c->timed_name.empty() || c->timed_name != name) return nullptr;
     if (c->ev_used + 2 > (int)c->ev_pool.size()) {
         if (c->ev_pool.size() >= 8192) return nullptr;
         for (int k = 0; k < 64; ++k) {
@@ -113,6 +115,8 @@ hipEvent_t* timing_slot(sld_ctx* c, const char* name) {
         }
     }
     hipEvent_t* ev = &c->ev_pool[c->ev_used];
+    c->ev_launches.resize(c->ev_used / 2 + 1);
+    c->ev_launches[c->ev_used / 2] = 1;
     c->ev_used += 2;
     return ev;
 }
@@ -333,18 +337,35 @@ int do_dTe(sld_ctx* c, double dt) {
     double pref = dtl / 2.0 * kappa * vf / kBoltz;
     int nb = nblk(c->D.C, 256);
     LAUNCH(c, k_dte_ab, nb, 256, h, c->phys, c->D, c->d_part);
+    // block partials: [0, nb) starting energies (k_dte_ab / k_dte_finish_ab),
+    // then the final energies of each backward level in turn
     for (int it = 0; it < nloop; ++it) {
-        TRY(reduce(c, nb, 1, c->d_scal + 8));
+        int npb = 0;
+        // the sweep levels are a few microseconds each: time them as one span
+        hipEvent_t* ev = timing_slot(c, "k_dte_sweep");
+        if (ev) {
+            (void)hipEventRecord(ev[0], c->stream);
+            c->ev_span = true;
+        }
+        int nlaunch = 0;
         for (int dir = 0; dir < 2; ++dir) {
             LevelSet& L = dir == 0 ? c->dte_fwd : c->dte_bwd;
             for (int l = 0; l < L.nlev(); ++l) {
                 int n = L.off[l + 1] - L.off[l];
+                double* part = dir == 1 ? c->d_part + (size_t)nb + npb : nullptr;
                 LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, pref, c->dx2,
-                       c->dy2, c->dz2, c->D);
+                       c->dy2, c->dz2, c->box, c->D, part);
+                if (dir == 1) npb += nblk(n, 256);
+                nlaunch += n > 0;
             }
         }
-        LAUNCH(c, k_dte_end, nb, 256, c->D, c->d_part + (size_t)nb);
-        TRY(reduce(c, nb, 1, c->d_scal + 9, (size_t)nb));
+        if (ev) {
+            c->ev_span = false;
+            (void)hipEventRecord(ev[1], c->stream);
+            c->ev_launches[(ev - c->ev_pool.data()) / 2] = nlaunch;
+        }
+        LAUNCH(c, k_reduce2, 1, 256, (const double*)c->d_part, nb, (const double*)(c->d_part + nb), npb,
+               c->d_scal + 8);
         if (it + 1 < nloop)
             LAUNCH(c, k_dte_finish_ab, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8), 1, c->d_part);
         else
@@ -594,8 +615,6 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
     TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
-    TRY(dalloc(c, &D.fn6, (size_t)6 * C));
-
     TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
     TRY(dalloc(c, &c->d_cl2, (size_t)C * kCellCap)); TRY(dalloc(c, &c->d_cn2, C));
     TRY(dalloc(c, &D.nb, (size_t)N * kMaxNb)); TRY(dalloc(c, &D.nbn, N));
@@ -669,17 +688,7 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
     std::vector<int> cslot(N);
     for (int cc = 0; cc < C; ++cc)
         for (int q = 0; q < cn[cc]; ++q) cslot[cl[(size_t)cc * kCellCap + q]] = cc * kCellCap + q;
-    std::vector<int> fn6((size_t)6 * C);
-    {
-        int nbh[26];
-        const int faces[6] = {0, 13, 2, 14, 12, 15};   // +x -x +y -y +z -z (map_cells.cpp:96-121)
-        for (int cc = 0; cc < C; ++cc) {
-            host_neigh(hc, cc, nbh);
-            for (int k = 0; k < 6; ++k) fn6[(size_t)6 * cc + k] = nbh[faces[k]];
-        }
-    }
     HIPCHK(c, hipMemcpyAsync(D.cslot, cslot.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(D.fn6, fn6.data(), fn6.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
 
     // groups and sweep levels
@@ -848,12 +857,13 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
     c->last_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->timed_launches = c->ev_used / 2;
+    c->timed_launches = 0;
     c->timed_ms = 0.0;
     for (int k = 0; k + 1 < c->ev_used; k += 2) {
         float kms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&kms, c->ev_pool[k], c->ev_pool[k + 1]));
         c->timed_ms += kms;
+        c->timed_launches += c->ev_launches[k / 2];
     }
     if (obs && k > 0)
         HIPCHK(c, hipMemcpy(obs, c->d_obs, sizeof(sld_obs) * k, hipMemcpyDeviceToHost));

@@ -18,7 +18,6 @@ struct Dev {
     int* cell;                 // link cell of each atom (new_cell_index)
     int* cell_prev;            // link cell before the last links()
     int* cslot;                // cell * kCellCap + position in the cell's list
-    int* fn6;                  // face neighbours of each cell [C][6]: +x -x +y -y +z -z
     int *cl, *cn;              // cell lists [C][kCellCap] head->tail, counts [C]
     int *nb, *nbn;             // neighbours within rcut_max [N][kMaxNb]
     int *nbm, *nbmn;           // neighbours within rcut_mag [N][kMaxMag]
@@ -38,8 +37,6 @@ struct Phys {
     int spinlang, lattlang, eltemp, localcolmot;
 };
 
-// Arguments of the persistent core_dTe kernel.
-
 // ---- kernels ----
 __global__ void k_bin(Box b, Dev D);
 __global__ void k_relink(Box b, const int* cl_old, const int* cn_old, Dev D);
@@ -55,8 +52,8 @@ __global__ void k_dr(double dt, Box b, Phys P, Dev D);
 __global__ void k_temperature(Tables T, Phys P, Dev D);
 __global__ void k_dte_ab(double h, Phys P, Dev D, double* partials);
 __global__ void k_dte_sweep(const int* cells, int ncell, double pref, double dx2, double dy2,
-                            double dz2, Dev D);
-__global__ void k_dte_end(Dev D, double* partials);
+                            double dz2, Box b, Dev D, double* partials);
+__global__ void k_reduce2(const double* pa, int na, const double* pb, int nb, double* out);
 __global__ void k_dte_finish(double h, Phys P, Dev D, const double* xp, int nxp);
 __global__ void k_dte_finish_ab(double h, Phys P, Dev D, const double* xp, int nxp, double* partials);
 __global__ void k_obs_atoms(Tables T, Phys P, Dev D, double* partials, int scan_spin);

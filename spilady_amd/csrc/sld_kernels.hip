@@ -801,28 +801,59 @@ __global__ __launch_bounds__(256) void k_dte_ab(double h, Phys P, Dev D, double*
 
 // one dependency level of the C diffusion sweep (core_dTe_CPU.cpp:155-197).
 // The reference recomputes E(Te) of the cell first; the stored Ee of the
-// cell's last update is that value up to one rounding.
-__global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref,
-                                                   double dx2, double dy2, double dz2, Dev D) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncell) return;
-    const int c = cells[t];
-    const int* f = D.fn6 + 6 * c;
-    const double* Te = D.Te;
-    const double tc = Te[c];
-    const double t0 = Te[f[0]], t1 = Te[f[1]], t2 = Te[f[2]], t3 = Te[f[3]], t4 = Te[f[4]],
-                 t5 = Te[f[5]];
-    double ee = D.Ee[c];
-    ee += pref * ((t0 + t1 - 2.0 * tc) / dx2 + (t2 + t3 - 2.0 * tc) / dy2 + (t4 + t5 - 2.0 * tc) / dz2);
-    D.Ee[c] = ee;
-    D.Te[c] = ee_to_te(ee);
+// cell's last update is that value up to one rounding. The six face
+// neighbours (map_cells.cpp slots 0/13, 2/14, 12/15) come from the cell index.
+// In the backward sweep every cell is updated exactly once, last, so the
+// launch also leaves block partials of the final energies (partials != null).
+__global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref, double dx2,
+                                                   double dy2, double dz2, Box b, Dev D, double* partials) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (t < ncell) {
+        const int c = cells[t];
+        const int nx = b.nx, ny = b.ny, nz = b.nz, sl = nx * ny;
+        const int ix = c % nx, iy = (c / nx) % ny, iz = c / sl;
+        const int xm = ix == 0 ? nx - 1 : ix - 1, xp = ix + 1 == nx ? 0 : ix + 1;
+        const int ym = iy == 0 ? ny - 1 : iy - 1, yp = iy + 1 == ny ? 0 : iy + 1;
+        const int zm = iz == 0 ? nz - 1 : iz - 1, zp = iz + 1 == nz ? 0 : iz + 1;
+        const int row = iy * nx + iz * sl;
+        const double* Te = D.Te;
+        const double tc = Te[c];
+        const double txm = Te[xm + row], txp = Te[xp + row];
+        const double tym = Te[ix + ym * nx + iz * sl], typ = Te[ix + yp * nx + iz * sl];
+        const double tzm = Te[ix + iy * nx + zm * sl], tzp = Te[ix + iy * nx + zp * sl];
+        double ee = D.Ee[c];
+        ee += pref * ((txp + txm - 2.0 * tc) / dx2 + (typ + tym - 2.0 * tc) / dy2 + (tzp + tzm - 2.0 * tc) / dz2);
+        D.Ee[c] = ee;
+        D.Te[c] = ee_to_te(ee);
+        acc[0] = ee;
+    }
+    if (partials) block_partials<1>(acc, partials);
 }
 
-// end of C: total electron energy from the cells' last updates
-__global__ __launch_bounds__(256) void k_dte_end(Dev D, double* partials) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[1] = {c < D.C ? D.Ee[c] : 0.0};
-    block_partials<1>(acc, partials);
+// the two totals of a sub-step: out[0] = sum of the starting-energy partials,
+// out[1] = sum of the final-energy partials
+__global__ __launch_bounds__(256) void k_reduce2(const double* pa, int na, const double* pb, int nb,
+                                                 double* out) {
+    __shared__ double sh[2][256];
+    const int t = threadIdx.x;
+    double a = 0.0, e = 0.0;
+    for (int k = t; k < na; k += 256) a += pa[k];
+    for (int k = t; k < nb; k += 256) e += pb[k];
+    sh[0][t] = a;
+    sh[1][t] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            sh[0][t] += sh[0][t + s];
+            sh[1][t] += sh[1][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+    }
 }
 
 // A(h) when E(Te) is already known (the previous A left it in Ee)
