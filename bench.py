@@ -95,6 +95,52 @@ def build_workload(n_unit: int):
     return v, p, at
 
 
+def batch_workload(nrep: int = 64, seed: int = 3):
+    """64 independent bcc Fe replicas: L^3 cubic cells with L drawn uniformly
+    from {8,12,...,32}, temperatures evenly spaced over 100-1400 K, seed
+    100 + index, Langevin lattice and spin thermostats, dt = 1 fs."""
+    from spilady_amd import abi, configs
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([8, 12, 16, 20, 24, 28, 32], size=nrep)
+    temps = np.linspace(100.0, 1400.0, nrep)
+    out = []
+    for r in range(nrep):
+        at = configs.make_bcc_fe(int(sizes[r]), float(temps[r]), 100 + r)
+        v = configs.fe_variables(float(temps[r]), 100 + r, thermostats=True, print_every=1)
+        out.append((abi.params_from_variables(v, at.box, at.natom, replica=r), at))
+    return out
+
+
+def batched_vs_serial(steps: int, warmup: int):
+    """The same replicas advanced in one batched context and one after the
+    other in single-system contexts; steps timed after warmup in both."""
+    from spilady_amd.engine import BatchEngine, Engine
+    reps = batch_workload()
+    natoms = sum(p.natom for p, _ in reps)
+    b = BatchEngine([p for p, _ in reps], [a for _, a in reps], compute_initial_forces=True)
+    b.step(0, warmup)
+    t0 = time.perf_counter()
+    ob = b.step(warmup, steps)
+    tb = time.perf_counter() - t0
+    b.close()
+    ts = 0.0
+    equal = True
+    for r, (p, a) in enumerate(reps):
+        e = Engine(p, a, compute_initial_forces=True)
+        e.step(0, warmup)
+        t0 = time.perf_counter()
+        os_ = e.step(warmup, steps)
+        ts += time.perf_counter() - t0
+        e.close()
+        equal = equal and all(np.array_equal(ob[k][r][x], os_[k][x]) for k in range(steps) for x in os_[k])
+    return {"replicas": len(reps), "atoms": natoms, "steps": steps,
+            "batched_atom_steps_per_s": natoms * steps / tb,
+            "serial_atom_steps_per_s": natoms * steps / ts,
+            "speedup": ts / tb, "observables_bitwise_equal": equal,
+            "workload": "64 bcc Fe replicas, L^3 cubic cells, L uniform in {8..32}, 100-1400 K, "
+                        "seed 100+r, Langevin lattice+spin, dt 1 fs"}
+
+
 def cpu_baseline(v, st_dict, box, natom, steps: int):
     """The compiled reference (oracle/_ref) on the same atoms, timed on the
     host's cores for a bounded number of steps; falls back to the
@@ -141,6 +187,8 @@ def main() -> int:
                     help="kernels timed with HIP events in extra steps after the timed region")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="timed steps of the batched-vs-serial replica comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the spin-lattice engine is single-device in this build "
@@ -207,6 +255,7 @@ def main() -> int:
                  "fp64_frac": value * STEP_FLOP / 1e12 / PEAK_FP64_TFLOPS,
                  "hbm_gbs": value * STEP_BYTES / 1e9,
                  "hbm_frac": value * STEP_BYTES / 1e9 / PEAK_HBM_GBS}
+    batched = batched_vs_serial(args.batch_steps, 2) if args.batch_steps > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(v, st, at.box, n, args.cpu_steps)
@@ -231,6 +280,7 @@ def main() -> int:
         "roofline": roof,
         "kernels": kern,
         "step_roofline": step_roof,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     if cpu:

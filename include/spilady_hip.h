@@ -117,6 +117,21 @@ int sld_device_info(char* name, int len, int* n_cu);
  * compute_initial_forces = 1 evaluates them instead (the non-readconf path). */
 int sld_create(const sld_params* params, const sld_atoms* init,
                int compute_initial_forces, sld_ctx** out);
+/* Batched replicas: nrep independent systems (params[r], init[r]) of any
+ * size, temperature and seed in one context; every step entry point advances
+ * all of them, each kernel launch covering all replicas. No counterpart in the
+ * reference, which runs one system per process; each replica follows the same
+ * reference functions as sld_create's single system, and its trajectory is
+ * bitwise the one sld_create gives it alone (noise is keyed by
+ * (random_seed, replica) and the replica's own atom index). The replicas must
+ * share dt, mass, damping, cutoffs, print interval and switches; eltemp needs
+ * nrep == 1. sld_create(p, a, f, out) is sld_create_batch(1, p, a, f, out). */
+int sld_create_batch(int nrep, const sld_params* params, const sld_atoms* init,
+                     int compute_initial_forces, sld_ctx** out);
+int sld_nreplicas(const sld_ctx* ctx);
+/* The replica that sld_download, sld_upload, sld_get_cells, sld_set_cells_te,
+ * sld_get_geometry and sld_get_cell_lists act on (default 0). */
+int sld_select_replica(sld_ctx* ctx, int replica);
 /* Replaces free_memory (free_memory.cpp:31-52). */
 int sld_destroy(sld_ctx* ctx);
 const char* sld_last_error(const sld_ctx* ctx);
@@ -139,16 +154,17 @@ int sld_core_dTe(sld_ctx* ctx, double dt);
 int sld_calculate_temperature(sld_ctx* ctx);
 /* check (check.cpp:31-47): check_energy (with the renormalizeEnergy feedback,
  * check_energy_CPU.cpp:77-111), check_temperature, check_spin. step = -1
- * records the initial energy. Writes the observables to *out (may be NULL). */
+ * records the initial energy. Writes one record per replica to out[0..nrep)
+ * (may be NULL). */
 int sld_check(sld_ctx* ctx, int step, sld_obs* out);
 /* nsteps of core() (core.cpp:31-76), each followed by total_time += step and
  * check() on printed steps, as the driver loop does (spilady.cpp:50-57).
- * obs (may be NULL) receives one entry per printed step, in order; *nobs
- * (may be NULL) the number written. */
+ * obs (may be NULL) receives nrep records per printed step, printed-step
+ * major (obs[k * nrep + r]); *nobs (may be NULL) the number of printed steps. */
 int sld_step(sld_ctx* ctx, int first_step, int nsteps, sld_obs* obs, int* nobs);
 
-/* Device <-> host state (copy_atoms_from_GPU_to_CPU, copy_CPU_btw_GPU.cu:141-150).
- * Atoms are in the caller's original index order. */
+/* Device <-> host state of the selected replica (copy_atoms_from_GPU_to_CPU,
+ * copy_CPU_btw_GPU.cu:141-150). Atoms are in the caller's original index order. */
 int sld_download(sld_ctx* ctx, sld_atoms* out);
 /* Overwrites r, p, s (and f, rho when non-NULL) and rebuilds nothing else:
  * the cell lists keep their current order. */

@@ -32,20 +32,36 @@ struct LevelSet {
     int nlev() const { return (int)off.size() - 1; }
 };
 
+struct HostCells {
+    int nx, ny, nz, C;
+};
+
+// host-side record of one replica
+struct ReplicaHost {
+    sld_params prm{};
+    HostCells hc{};
+    std::vector<int> group_size;
+    std::vector<int> group_cells;  // concatenated in group order (local cell ids)
+    double vol = 0.0, dx2 = 0.0, dy2 = 0.0, dz2 = 0.0;
+};
+
 }  // namespace
 
 struct sld_ctx {
-    sld_params prm{};
-    Box box{};
+    sld_params prm{};               // parameters shared by all replicas
+    int R = 1;                      // replicas
+    int sel = 0;                    // replica that the state accessors act on
+    std::vector<ReplicaHost> rh;    // [R]
+    std::vector<Rep> reps;          // [R], mirrored in D.rep
+    std::vector<int> atom_seg;      // 256-slot blocks of each replica [R + 1]
+    std::vector<int> cell8_seg;     // 8-cell blocks of each replica [R + 1]
+    int *d_atom_seg = nullptr, *d_cell8_seg = nullptr;
     Phys phys{};
     Tables tab{};
     Dev D{};
     hipStream_t stream = nullptr;
     int device = 0;
-    int ngroups = 0;
-    std::vector<int> group_size;
-    std::vector<int> group_cells;  // concatenated in group order
-    LevelSet spin_fwd, spin_bwd, dte_fwd, dte_bwd;
+    LevelSet spin_fwd, spin_bwd, dte_fwd, dte_bwd;  // over all replicas
     // buffers
     double *d_rtab = nullptr, *d_rhotab = nullptr;
     int *d_cl2 = nullptr, *d_cn2 = nullptr;  // second cell-list buffer
@@ -57,9 +73,7 @@ struct sld_ctx {
     double total_time = 0.0;
     uint64_t pass = 0;                       // noise pass counter
     bool nb_dirty = true;
-    double vol = 0.0, dx2 = 0, dy2 = 0, dz2 = 0;
     double last_ms = 0.0;
-    // persistent core_dTe
     // per-kernel event timing of one named kernel
     std::string timed_name;
     std::vector<hipEvent_t> ev_pool;
@@ -142,10 +156,6 @@ hipEvent_t* timing_slot(sld_ctx* c, const char* name) {
 
 // ---------------------------------------------------------------------------
 // host-side set-up: cells, lists, groups, levels
-
-struct HostCells {
-    int nx, ny, nz, C;
-};
 
 // map_cells.cpp:41-60
 HostCells cell_counts(const double* d, double mlc) {
@@ -260,8 +270,14 @@ int reduce(sld_ctx* c, int nblocks, int nk, double* out, size_t part_offset = 0)
     return SLD_OK;
 }
 
+// per-replica sums of block partials into each replica's scalar block
+int reduce_seg(sld_ctx* c, const int* d_seg, int nk, int out_off) {
+    LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)c->d_part, d_seg, nk, c->d_scal, out_off);
+    return SLD_OK;
+}
+
 int nbuild(sld_ctx* c, int write_rho) {
-    LAUNCH(c, k_nbuild, nblk(c->D.C, 4), 256, c->box, c->tab, c->phys, c->D, write_rho);
+    LAUNCH(c, k_nbuild, nblk(c->D.C, 4), 256, c->tab, c->phys, c->D, write_rho);
     c->nb_dirty = false;
     return SLD_OK;
 }
@@ -272,18 +288,18 @@ int ensure_nb(sld_ctx* c) {
 }
 
 int do_links(sld_ctx* c) {
-    LAUNCH(c, k_bin, nblk(c->D.N, 256), 256, c->box, c->D);
+    LAUNCH(c, k_bin, nblk(c->D.N, 256), 256, c->D);
     // swap list buffers: the old lists become the input of the rebuild
     std::swap(c->D.cl, c->d_cl2);
     std::swap(c->D.cn, c->d_cn2);
-    LAUNCH(c, k_relink, nblk(c->D.C, 128), 128, c->box, c->d_cl2, c->d_cn2, c->D);
+    LAUNCH(c, k_relink, nblk(c->D.C, 128), 128, (const int*)c->d_cl2, (const int*)c->d_cn2, c->D);
     return SLD_OK;
 }
 
 int do_force(sld_ctx* c) {
     TRY(ensure_nb(c));
     LAUNCH(c, k_fp, nblk(c->D.N, 256), 256, c->tab, c->D);
-    LAUNCH(c, k_force, nblk(c->D.N, 256), 256, c->box, c->tab, c->phys, c->D);
+    LAUNCH(c, k_force, nblk(c->D.N, 256), 256, c->tab, c->phys, c->D);
     return SLD_OK;
 }
 
@@ -302,20 +318,22 @@ int do_ds(sld_ctx* c, double dt) {
     return SLD_OK;
 }
 
+// scalar block of each replica: [0..2] momentum sum, [3..4] kinetic energy
+// before and after centring
 int do_dp(sld_ctx* c, double dt) {
     int nb = nblk(c->D.C, 8);
     LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, c->pass, c->d_part);
     if (c->phys.lattlang) c->pass++;
-    TRY(reduce(c, nb, 3, c->d_scal));
+    TRY(reduce_seg(c, c->d_cell8_seg, 3, 0));
     int na = nblk(c->D.N, 256);
     LAUNCH(c, k_dp_center, na, 256, c->phys, c->D, (const double*)c->d_scal, c->d_part);
-    TRY(reduce(c, na, 2, c->d_scal + 3));
-    LAUNCH(c, k_dp_scale, na, 256, c->phys, c->D, (const double*)(c->d_scal + 3));
+    TRY(reduce_seg(c, c->d_atom_seg, 2, 3));
+    LAUNCH(c, k_dp_scale, na, 256, c->phys, c->D, (const double*)c->d_scal);
     return SLD_OK;
 }
 
 int do_dr(sld_ctx* c, double dt) {
-    LAUNCH(c, k_dr, nblk(c->D.N, 256), 256, dt, c->box, c->phys, c->D);
+    LAUNCH(c, k_dr, nblk(c->D.N, 256), 256, dt, c->phys, c->D);
     c->nb_dirty = true;
     return SLD_OK;
 }
@@ -331,9 +349,12 @@ int do_dTe(sld_ctx* c, double dt) {
     const int nloop = 50;
     double dtl = dt / nloop;
     double h = dtl / 2.0;
+    // the electron subsystem runs on single-system contexts only
+    const ReplicaHost& h0 = c->rh[0];
+    const double natom = c->reps[0].natom, ncell = c->reps[0].ncell;
     double kappa = c->prm.kappa_e / kEvToJ / 1e10;
-    double atomic_volume = c->vol / (double)c->D.N;
-    double vf = atomic_volume / ((double)c->D.N / (double)c->D.C);
+    double atomic_volume = h0.vol / natom;
+    double vf = atomic_volume / (natom / ncell);
     double pref = dtl / 2.0 * kappa * vf / kBoltz;
     int nb = nblk(c->D.C, 256);
     LAUNCH(c, k_dte_ab, nb, 256, h, c->phys, c->D, c->d_part);
@@ -353,8 +374,8 @@ int do_dTe(sld_ctx* c, double dt) {
             for (int l = 0; l < L.nlev(); ++l) {
                 int n = L.off[l + 1] - L.off[l];
                 double* part = dir == 1 ? c->d_part + (size_t)nb + npb : nullptr;
-                LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, pref, c->dx2,
-                       c->dy2, c->dz2, c->box, c->D, part);
+                LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, pref, h0.dx2,
+                       h0.dy2, h0.dz2, c->reps[0].box, c->D, part);
                 if (dir == 1) npb += nblk(n, 256);
                 nlaunch += n > 0;
             }
@@ -379,9 +400,16 @@ int do_dTe(sld_ctx* c, double dt) {
 // observables record from the reduced sums; renormalizeEnergy error
 // (check_energy_CPU.cpp:33-111, check_temperature_CPU.cpp:33-134,
 // check_spin_CPU.cpp:33-56)
-__global__ void k_obs_final(const double* sa, const double* sc, int N, int C, int eltemp, int renorm,
-                            int step, double total_time, double* einit, sld_obs* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One thread per replica; its scalar block holds the atom sums at [16..27]
+// and the cell sums at [32..37]. renormalizeEnergy (single system) keeps the
+// initial energy and the error in einit[0..1].
+__global__ void k_obs_final(const double* scal, const Rep* rep, int R, int eltemp, int renorm, int step,
+                            double total_time, double* einit, sld_obs* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const double* sa = scal + (size_t)r * kScal + 16;
+    const double* sc = scal + (size_t)r * kScal + 32;
+    const int N = rep[r].natom, C = rep[r].ncell;
     sld_obs o = {};
     o.step = step;
     o.total_time = total_time;
@@ -413,7 +441,7 @@ __global__ void k_obs_final(const double* sa, const double* sc, int N, int C, in
     o.ave_m[0] = -kElG * sx; o.ave_m[1] = -kElG * sy; o.ave_m[2] = -kElG * sz;
     o.ave_m0 = sqrt(o.ave_m[0] * o.ave_m[0] + o.ave_m[1] * o.ave_m[1] + o.ave_m[2] * o.ave_m[2]);
     o.virial = sa[11];
-    *out = o;
+    out[r] = o;
 }
 
 __global__ void k_obs_set_te(const double* te_sum, int C, sld_obs* out) {
@@ -427,15 +455,14 @@ int do_check(sld_ctx* c, int step, sld_obs* d_out) {
     int scan = c->phys.eltemp ? 0 : 1;
     if (scan) TRY(ensure_nb(c));
     LAUNCH(c, k_obs_atoms, na, 256, c->tab, c->phys, c->D, c->d_part, scan);
-    TRY(reduce(c, na, kObsAtomK + 1, c->d_scal + 16));
+    TRY(reduce_seg(c, c->d_atom_seg, kObsAtomK + 1, 16));
     if (c->phys.eltemp) {
         int nb = nblk(c->D.C, 256);
         LAUNCH(c, k_obs_cells, nb, 256, c->D, c->d_part);
         TRY(reduce(c, nb, kObsCellK, c->d_scal + 32));
     }
-    LAUNCH(c, k_obs_final, 1, 64, (const double*)(c->d_scal + 16), (const double*)(c->d_scal + 32),
-           c->D.N, c->D.C, c->phys.eltemp, c->prm.renormalize_energy, step, c->total_time,
-           c->d_einit, d_out);
+    LAUNCH(c, k_obs_final, nblk(c->R, 64), 64, (const double*)c->d_scal, (const Rep*)c->D.rep, c->R,
+           c->phys.eltemp, c->prm.renormalize_energy, step, c->total_time, c->d_einit, d_out);
     if (c->phys.eltemp && c->prm.renormalize_energy) {
         int nb = nblk(c->D.C, 256);
         LAUNCH(c, k_renorm, nb, 256, c->D, (const double*)(c->d_einit + 1), c->d_part);
@@ -480,135 +507,215 @@ int read_flags(sld_ctx* c) {
     return SLD_OK;
 }
 
-void pack4(const double* src, std::vector<double>& dst, int n, bool with_len) {
-    dst.assign((size_t)n * 4, 0.0);
-    for (int i = 0; i < n; ++i) {
+// host <-> device copies of replica `r`'s atom slots [atom0, atom0 + natom)
+int up4(sld_ctx* c, double4* d, const double* src, int r, bool with_len) {
+    if (!src) return SLD_OK;
+    const Rep& R = c->reps[r];
+    std::vector<double> h((size_t)R.natom * 4);
+    for (int i = 0; i < R.natom; ++i) {
         double x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
-        dst[4 * i] = x; dst[4 * i + 1] = y; dst[4 * i + 2] = z;
-        dst[4 * i + 3] = with_len ? std::sqrt(x * x + y * y + z * z) : 0.0;
+        h[4 * i] = x; h[4 * i + 1] = y; h[4 * i + 2] = z;
+        h[4 * i + 3] = with_len ? std::sqrt(x * x + y * y + z * z) : 0.0;
     }
-}
-
-int up4(sld_ctx* c, double4* d, const double* src, bool with_len) {
-    if (!src) return SLD_OK;
-    std::vector<double> h;
-    pack4(src, h, c->D.N, with_len);
-    HIPCHK(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d + R.atom0, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SLD_OK;
 }
 
-int up1(sld_ctx* c, double* d, const double* src, size_t n) {
+int up1(sld_ctx* c, double* d, const double* src, size_t off, size_t n) {
     if (!src) return SLD_OK;
-    HIPCHK(c, hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d + off, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SLD_OK;
 }
 
-int down4(sld_ctx* c, const double4* d, double* dst) {
+int down4(sld_ctx* c, const double4* d, double* dst, int r) {
     if (!dst) return SLD_OK;
-    std::vector<double> h((size_t)c->D.N * 4);
-    HIPCHK(c, hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const Rep& R = c->reps[r];
+    std::vector<double> h((size_t)R.natom * 4);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d + R.atom0, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int i = 0; i < c->D.N; ++i) {
+    for (int i = 0; i < R.natom; ++i) {
         dst[3 * i] = h[4 * i]; dst[3 * i + 1] = h[4 * i + 1]; dst[3 * i + 2] = h[4 * i + 2];
     }
     return SLD_OK;
 }
 
-int down1(sld_ctx* c, const double* d, double* dst, size_t n) {
+int down1(sld_ctx* c, const double* d, double* dst, size_t off, size_t n) {
     if (!dst) return SLD_OK;
-    HIPCHK(c, hipMemcpyAsync(dst, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst, d + off, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
+template <class T>
+int upload_vec(sld_ctx* c, T** d, const std::vector<T>& h) {
+    TRY(dalloc(c, d, h.size()));
+    HIPCHK(c, hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SLD_OK;
 }
 
 int upload_levels(sld_ctx* c, LevelSet& L) {
-    TRY(dalloc(c, &L.d_cells, L.cells.size()));
-    TRY(dalloc(c, &L.d_off, L.off.size()));
-    HIPCHK(c, hipMemcpyAsync(L.d_cells, L.cells.data(), L.cells.size() * sizeof(int),
-                             hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(L.d_off, L.off.data(), L.off.size() * sizeof(int),
-                             hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    TRY(upload_vec(c, &L.d_cells, L.cells));
+    TRY(upload_vec(c, &L.d_off, L.off));
     return SLD_OK;
 }
 
-int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int compute_forces) {
-    c->prm = *p;
-    sld_params& P = c->prm;
-    if (P.natom <= 0) return set_err(c, SLD_ERR_ARG, "natom must be positive");
-    if (!in || !in->r || !in->p || !in->s) return set_err(c, SLD_ERR_ARG, "r, p and s are required");
-    if (!(P.box[0] > 0 && P.box[2] > 0 && P.box[5] > 0)) return set_err(c, SLD_ERR_ARG, "box diagonal must be positive");
-    if (P.rcut_max < P.rcut_pot || P.rcut_max < P.rcut_mag)
-        return set_err(c, SLD_ERR_ARG, "rcut_max must be the largest cutoff");
-    if (P.rcut_max > P.min_length_link_cell) P.min_length_link_cell = P.rcut_max + 0.001;
-    if (P.localcolmot && !P.eltemp) P.localcolmot = 0;
-    if (P.renormalize_energy && !P.eltemp) P.renormalize_energy = 0;
-    if (P.step <= 0) return set_err(c, SLD_ERR_ARG, "step must be positive");
+// the level schedules of all replicas as one: level l holds every replica's
+// level-l cells (replicas never interact)
+void merge_levels(const std::vector<LevelSet>& per, const std::vector<Rep>& reps, LevelSet& out) {
+    int nl = 0;
+    for (const LevelSet& L : per) nl = std::max(nl, L.nlev());
+    out.cells.clear();
+    out.off.assign(1, 0);
+    for (int l = 0; l < nl; ++l) {
+        for (size_t r = 0; r < per.size(); ++r) {
+            const LevelSet& L = per[r];
+            if (l >= L.nlev()) continue;
+            for (int k = L.off[l]; k < L.off[l + 1]; ++k) out.cells.push_back(reps[r].cell0 + L.cells[k]);
+        }
+        out.off.push_back((int)out.cells.size());
+    }
+}
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// the parameters every replica of a batch must share (one table set, one dt,
+// one set of switches)
+bool same_shared(const sld_params& a, const sld_params& b) {
+    return a.step == b.step && a.atmass == b.atmass && a.gamma_L_over_mass == b.gamma_L_over_mass &&
+           a.gamma_S_HL == b.gamma_S_HL && a.kappa_e == b.kappa_e && a.rcut_pot == b.rcut_pot &&
+           a.rcut_mag == b.rcut_mag && a.rcut_max == b.rcut_max &&
+           a.min_length_link_cell == b.min_length_link_cell && a.start_time == b.start_time &&
+           a.interval_of_print_out == b.interval_of_print_out && a.spinlang == b.spinlang &&
+           a.lattlang == b.lattlang && a.eltemp == b.eltemp && a.localcolmot == b.localcolmot &&
+           a.renormalize_energy == b.renormalize_energy;
+}
+
+int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms* init, int compute_forces) {
+    if (nrep <= 0) return set_err(c, SLD_ERR_ARG, "need at least one replica");
+    c->R = nrep;
+    c->rh.resize(nrep);
+    c->reps.resize(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        sld_params P = params[r];
+        const sld_atoms* in = init + r;
+        if (P.natom <= 0) return set_err(c, SLD_ERR_ARG, "replica %d: natom must be positive", r);
+        if (!in->r || !in->p || !in->s) return set_err(c, SLD_ERR_ARG, "replica %d: r, p and s are required", r);
+        if (!(P.box[0] > 0 && P.box[2] > 0 && P.box[5] > 0))
+            return set_err(c, SLD_ERR_ARG, "replica %d: box diagonal must be positive", r);
+        if (P.rcut_max < P.rcut_pot || P.rcut_max < P.rcut_mag)
+            return set_err(c, SLD_ERR_ARG, "rcut_max must be the largest cutoff");
+        if (P.rcut_max > P.min_length_link_cell) P.min_length_link_cell = P.rcut_max + 0.001;
+        if (P.localcolmot && !P.eltemp) P.localcolmot = 0;
+        if (P.renormalize_energy && !P.eltemp) P.renormalize_energy = 0;
+        if (P.step <= 0) return set_err(c, SLD_ERR_ARG, "step must be positive");
+        if (r > 0 && !same_shared(P, c->rh[0].prm))
+            return set_err(c, SLD_ERR_ARG, "replica %d: dt, mass, damping, cutoffs, print interval and switches "
+                                           "must equal replica 0's", r);
+        c->rh[r].prm = P;
+    }
+    c->prm = c->rh[0].prm;
+    const sld_params& P0 = c->prm;
+    if (nrep > 1 && P0.eltemp)
+        return set_err(c, SLD_ERR_ARG, "the electron-temperature model runs on single-system contexts only");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_err(c, SLD_ERR_NOGPU, "no HIP device");
     HIPCHK(c, hipGetDevice(&c->device));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
-    // geometry (read_config.cpp:110-116, vec_utils_CPU.cpp:135-148, map_cells.cpp:41-60)
-    const double* d = P.box;
-    Box& b = c->box;
-    for (int k = 0; k < 6; ++k) b.d[k] = d[k];
-    b.inv[0] = 1e0 / d[0];
-    b.inv[1] = -d[1] / (d[0] * d[2]);
-    b.inv[2] = 1e0 / d[2];
-    b.inv[3] = (d[1] * d[4] - d[2] * d[3]) / (d[0] * d[2] * d[5]);
-    b.inv[4] = -d[4] / (d[2] * d[5]);
-    b.inv[5] = 1e0 / d[5];
-    HostCells hc = cell_counts(d, P.min_length_link_cell);
-    if (hc.nx < 3 || hc.ny < 3 || hc.nz < 3)
-        return set_err(c, SLD_ERR_ARG, "need at least 3 link cells per side (got %d %d %d)", hc.nx, hc.ny, hc.nz);
-    b.nx = hc.nx; b.ny = hc.ny; b.nz = hc.nz;
-    const int N = P.natom, C = hc.C;
-    c->vol = d[0] * d[2] * d[5];
-    double blx = std::fabs(d[0]);
-    double bly = std::sqrt(d[1] * d[1] + d[2] * d[2]);
-    double blz = std::sqrt(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
-    c->dx2 = std::pow(blx / hc.nx, 2);
-    c->dy2 = std::pow(bly / hc.ny, 2);
-    c->dz2 = std::pow(blz / hc.nz, 2);
+    // geometry of every replica (read_config.cpp:110-116, vec_utils_CPU.cpp:135-148,
+    // map_cells.cpp:41-60) and its place in the concatenated arrays
+    int N = 0, C = 0;
+    c->atom_seg.assign(nrep + 1, 0);
+    c->cell8_seg.assign(nrep + 1, 0);
+    for (int r = 0; r < nrep; ++r) {
+        ReplicaHost& h = c->rh[r];
+        const double* d = h.prm.box;
+        Rep& R = c->reps[r];
+        Box& b = R.box;
+        for (int k = 0; k < 6; ++k) b.d[k] = d[k];
+        b.inv[0] = 1e0 / d[0];
+        b.inv[1] = -d[1] / (d[0] * d[2]);
+        b.inv[2] = 1e0 / d[2];
+        b.inv[3] = (d[1] * d[4] - d[2] * d[3]) / (d[0] * d[2] * d[5]);
+        b.inv[4] = -d[4] / (d[2] * d[5]);
+        b.inv[5] = 1e0 / d[5];
+        h.hc = cell_counts(d, h.prm.min_length_link_cell);
+        if (h.hc.nx < 3 || h.hc.ny < 3 || h.hc.nz < 3)
+            return set_err(c, SLD_ERR_ARG, "replica %d: need at least 3 link cells per side (got %d %d %d)", r,
+                           h.hc.nx, h.hc.ny, h.hc.nz);
+        b.nx = h.hc.nx; b.ny = h.hc.ny; b.nz = h.hc.nz;
+        h.vol = d[0] * d[2] * d[5];
+        const double blx = std::fabs(d[0]);
+        const double bly = std::sqrt(d[1] * d[1] + d[2] * d[2]);
+        const double blz = std::sqrt(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+        h.dx2 = std::pow(blx / h.hc.nx, 2);
+        h.dy2 = std::pow(bly / h.hc.ny, 2);
+        h.dz2 = std::pow(blz / h.hc.nz, 2);
+        R.temperature = h.prm.temperature;
+        R.box_volume = h.vol;
+        R.seed = h.prm.random_seed;
+        R.replica = h.prm.replica;
+        R.natom = h.prm.natom;
+        R.atom0 = round_up(N, 256);
+        R.ncell = h.hc.C;
+        R.cell0 = round_up(C, 8);
+        N = R.atom0 + R.natom;
+        C = R.cell0 + R.ncell;
+        c->atom_seg[r] = R.atom0 / 256;
+        c->cell8_seg[r] = R.cell0 / 8;
+    }
+    c->atom_seg[nrep] = nblk(N, 256);
+    c->cell8_seg[nrep] = nblk(C, 8);
 
     Phys& ph = c->phys;
-    ph.atmass = P.atmass;
-    ph.gamma_L = P.gamma_L_over_mass * P.atmass;
-    ph.gamma_S = P.gamma_S_HL;
-    ph.temperature = P.temperature;
-    ph.rcut_pot = P.rcut_pot;
-    ph.rcut_mag = P.rcut_mag;
-    ph.rcut_pot_sq = std::pow(P.rcut_pot, 2);
-    ph.rcut_mag_sq = std::pow(P.rcut_mag, 2);
-    ph.rcut_max_sq = std::pow(P.rcut_max, 2);
-    ph.box_volume = c->vol;
-    ph.seed = P.random_seed;
-    ph.replica = P.replica;
-    ph.spinlang = P.spinlang;
-    ph.lattlang = P.lattlang;
-    ph.eltemp = P.eltemp;
-    ph.localcolmot = P.localcolmot;
+    ph.atmass = P0.atmass;
+    ph.gamma_L = P0.gamma_L_over_mass * P0.atmass;
+    ph.gamma_S = P0.gamma_S_HL;
+    ph.rcut_pot = P0.rcut_pot;
+    ph.rcut_mag = P0.rcut_mag;
+    ph.rcut_pot_sq = std::pow(P0.rcut_pot, 2);
+    ph.rcut_mag_sq = std::pow(P0.rcut_mag, 2);
+    ph.rcut_max_sq = std::pow(P0.rcut_max, 2);
+    ph.spinlang = P0.spinlang;
+    ph.lattlang = P0.lattlang;
+    ph.eltemp = P0.eltemp;
+    ph.localcolmot = P0.localcolmot;
 
-    // tables
-    HostTables ht;
-    build_tables(P.rcut_max, ht);
-    TRY(dalloc(c, &c->d_rtab, ht.rtab.size()));
-    TRY(dalloc(c, &c->d_rhotab, ht.rhotab.size()));
-    HIPCHK(c, hipMemcpyAsync(c->d_rtab, ht.rtab.data(), ht.rtab.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_rhotab, ht.rhotab.data(), ht.rhotab.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // tables (shared)
+    std::shared_ptr<const HostTables> ht = shared_tables(P0.rcut_max);
+    TRY(upload_vec(c, &c->d_rtab, ht->rtab));
+    TRY(upload_vec(c, &c->d_rhotab, ht->rhotab));
     c->tab.rtab = c->d_rtab;
     c->tab.rhotab = c->d_rhotab;
-    c->tab.in_over_rmax = ht.in_over_rmax;
-    c->tab.in_over_rhomax = ht.in_over_rhomax;
+    c->tab.in_over_rmax = ht->in_over_rmax;
+    c->tab.in_over_rhomax = ht->in_over_rhomax;
 
     // device arrays
     Dev& D = c->D;
     D.N = N;
     D.C = C;
+    {
+        std::vector<int> arep(N, -1), crep(C, -1);
+        for (int r = 0; r < nrep; ++r) {
+            const Rep& R = c->reps[r];
+            std::fill(arep.begin() + R.atom0, arep.begin() + R.atom0 + R.natom, r);
+            std::fill(crep.begin() + R.cell0, crep.begin() + R.cell0 + R.ncell, r);
+        }
+        Rep* d_rep = nullptr;
+        int *d_arep = nullptr, *d_crep = nullptr;
+        TRY(upload_vec(c, &d_rep, c->reps));
+        TRY(upload_vec(c, &d_arep, arep));
+        TRY(upload_vec(c, &d_crep, crep));
+        D.rep = d_rep;
+        D.arep = d_arep;
+        D.crep = d_crep;
+        TRY(upload_vec(c, &c->d_atom_seg, c->atom_seg));
+        TRY(upload_vec(c, &c->d_cell8_seg, c->cell8_seg));
+    }
     TRY(dalloc(c, &D.r, N)); TRY(dalloc(c, &D.p, N)); TRY(dalloc(c, &D.f, N));
     TRY(dalloc(c, &D.s, N)); TRY(dalloc(c, &D.hh, N)); TRY(dalloc(c, &D.hl, N));
     TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
@@ -627,100 +734,109 @@ int create_impl(sld_ctx* c, const sld_params* p, const sld_atoms* in, int comput
     TRY(dalloc(c, &D.err, 1));
     int maxblk = std::max(nblk(N, 256), nblk(C, 8)) + 1;
     TRY(dalloc(c, &c->d_part, (size_t)maxblk * 16));
-    TRY(dalloc(c, &c->d_scal, 64));
+    TRY(dalloc(c, &c->d_scal, (size_t)kScal * nrep));
     TRY(dalloc(c, &c->d_einit, 4));
     c->obs_cap = 1;
-    TRY(dalloc(c, &c->d_obs, 1));
+    TRY(dalloc(c, &c->d_obs, nrep));
 
-    // atoms: positions wrapped as initial_links does (links_CPU.cpp:37-50)
-    std::vector<double> r3(in->r, in->r + 3 * (size_t)N);
-    std::vector<int> cell(N);
-    for (int i = 0; i < N; ++i) {
-        double x = r3[3 * i], y = r3[3 * i + 1], z = r3[3 * i + 2];
-        double q[3] = {b.inv[0] * x + b.inv[1] * y + b.inv[3] * z, b.inv[2] * y + b.inv[4] * z, b.inv[5] * z};
-        int idx = 0;
-        for (int k = 0; k < 3; ++k)
-            if (q[k] < 0e0 || q[k] >= 1e0) { q[k] -= (int)q[k]; ++idx; }
-        if (idx > 0) {
-            r3[3 * i] = d[0] * q[0] + d[1] * q[1] + d[3] * q[2];
-            r3[3 * i + 1] = d[2] * q[1] + d[4] * q[2];
-            r3[3 * i + 2] = d[5] * q[2];
+    // atoms of every replica: positions wrapped as initial_links does
+    // (links_CPU.cpp:37-50); initial lists with each atom pushed at the head in
+    // ascending index, so a list holds its atoms in descending index (:53-81)
+    std::vector<int> cell(N, 0), cn(C, 0), cl((size_t)C * kCellCap, 0), cslot(N, 0);
+    std::vector<LevelSet> sf(nrep), sb(nrep), tf(nrep), tb(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        const Rep& R = c->reps[r];
+        ReplicaHost& h = c->rh[r];
+        const sld_atoms* in = init + r;
+        const Box& b = R.box;
+        const double* d = h.prm.box;
+        const HostCells& hc = h.hc;
+        const int n = R.natom;
+        std::vector<double> r3(in->r, in->r + 3 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            double x = r3[3 * i], y = r3[3 * i + 1], z = r3[3 * i + 2];
+            double q[3] = {b.inv[0] * x + b.inv[1] * y + b.inv[3] * z, b.inv[2] * y + b.inv[4] * z, b.inv[5] * z};
+            int idx = 0;
+            for (int k = 0; k < 3; ++k)
+                if (q[k] < 0e0 || q[k] >= 1e0) { q[k] -= (int)q[k]; ++idx; }
+            if (idx > 0) {
+                r3[3 * i] = d[0] * q[0] + d[1] * q[1] + d[3] * q[2];
+                r3[3 * i + 1] = d[2] * q[1] + d[4] * q[2];
+                r3[3 * i + 2] = d[5] * q[2];
+            }
+            x = r3[3 * i]; y = r3[3 * i + 1]; z = r3[3 * i + 2];
+            double qx = b.inv[0] * x + b.inv[1] * y + b.inv[3] * z, qy = b.inv[2] * y + b.inv[4] * z,
+                   qz = b.inv[5] * z;
+            int lx = (int)(qx * hc.nx), ly = (int)(qy * hc.ny), lz = (int)(qz * hc.nz);
+            lx = std::min(std::max(lx, 0), hc.nx - 1);
+            ly = std::min(std::max(ly, 0), hc.ny - 1);
+            lz = std::min(std::max(lz, 0), hc.nz - 1);
+            cell[R.atom0 + i] = R.cell0 + lx + ly * hc.nx + lz * hc.nx * hc.ny;
         }
-        x = r3[3 * i]; y = r3[3 * i + 1]; z = r3[3 * i + 2];
-        double qx = b.inv[0] * x + b.inv[1] * y + b.inv[3] * z, qy = b.inv[2] * y + b.inv[4] * z, qz = b.inv[5] * z;
-        int lx = (int)(qx * hc.nx), ly = (int)(qy * hc.ny), lz = (int)(qz * hc.nz);
-        lx = std::min(std::max(lx, 0), hc.nx - 1);
-        ly = std::min(std::max(ly, 0), hc.ny - 1);
-        lz = std::min(std::max(lz, 0), hc.nz - 1);
-        cell[i] = lx + ly * hc.nx + lz * hc.nx * hc.ny;
+        TRY(up4(c, D.r, r3.data(), r, false));
+        TRY(up4(c, D.p, in->p, r, false));
+        TRY(up4(c, D.f, in->f, r, false));
+        TRY(up4(c, D.s, in->s, r, true));
+        TRY(up4(c, D.hh, in->heff_h, r, false));
+        TRY(up4(c, D.hl, in->heff_l, r, false));
+        TRY(up1(c, D.rho, in->rho, R.atom0, n));
+        TRY(up1(c, D.pe, in->pe, R.atom0, n));
+        TRY(up1(c, D.ke, in->ke, R.atom0, n));
+        TRY(up1(c, D.me, in->me, R.atom0, n));
+        TRY(up1(c, D.me0, in->me0, R.atom0, n));
+        if (in->stress)
+            for (int k = 0; k < 6; ++k) {
+                std::vector<double> st(n);
+                for (int i = 0; i < n; ++i) st[i] = in->stress[6 * i + k];
+                TRY(up1(c, D.stress, st.data(), (size_t)k * N + R.atom0, n));
+            }
+        for (int i = n - 1; i >= 0; --i) {
+            const int a = R.atom0 + i, cc = cell[a];
+            if (cn[cc] >= kCellCap)
+                return set_err(c, SLD_ERR_CAPACITY, "replica %d: a link cell exceeds %d atoms at start", r, kCellCap);
+            cl[(size_t)cc * kCellCap + cn[cc]++] = a;
+        }
+        // groups and sweep levels (allocate_cells_CPU.cpp:33-125)
+        std::vector<int> group_of;
+        greedy_groups(hc, h.group_size, h.group_cells, group_of);
+        int all26[26];
+        for (int k = 0; k < 26; ++k) all26[k] = k;
+        const int faces[6] = {0, 13, 2, 14, 12, 15};
+        make_levels(hc, h.group_size, h.group_cells, group_of, all26, 26, false, sf[r]);
+        make_levels(hc, h.group_size, h.group_cells, group_of, all26, 26, true, sb[r]);
+        make_levels(hc, h.group_size, h.group_cells, group_of, faces, 6, false, tf[r]);
+        make_levels(hc, h.group_size, h.group_cells, group_of, faces, 6, true, tb[r]);
     }
-    TRY(up4(c, D.r, r3.data(), false));
-    TRY(up4(c, D.p, in->p, false));
-    TRY(up4(c, D.f, in->f, false));
-    TRY(up4(c, D.s, in->s, true));
-    TRY(up4(c, D.hh, in->heff_h, false));
-    TRY(up4(c, D.hl, in->heff_l, false));
-    TRY(up1(c, D.rho, in->rho, N));
-    TRY(up1(c, D.pe, in->pe, N));
-    TRY(up1(c, D.ke, in->ke, N));
-    TRY(up1(c, D.me, in->me, N));
-    TRY(up1(c, D.me0, in->me0, N));
-    if (in->stress) {
-        std::vector<double> st((size_t)6 * N);
-        for (int i = 0; i < N; ++i)
-            for (int k = 0; k < 6; ++k) st[(size_t)k * N + i] = in->stress[6 * i + k];
-        HIPCHK(c, hipMemcpyAsync(D.stress, st.data(), st.size() * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-
-    // initial lists: each atom pushed at the head in ascending index, so a
-    // list holds its atoms in descending index (links_CPU.cpp:53-81)
-    std::vector<int> cn(C, 0), cl((size_t)C * kCellCap, 0);
-    for (int i = N - 1; i >= 0; --i) {
-        int cc = cell[i];
-        if (cn[cc] >= kCellCap) return set_err(c, SLD_ERR_CAPACITY, "a link cell exceeds %d atoms at start", kCellCap);
-        cl[(size_t)cc * kCellCap + cn[cc]++] = i;
-    }
+    for (int cc = 0; cc < C; ++cc)
+        for (int q = 0; q < cn[cc]; ++q) cslot[cl[(size_t)cc * kCellCap + q]] = cc * kCellCap + q;
     HIPCHK(c, hipMemcpyAsync(D.cell, cell.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.cell_prev, cell.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.cn, cn.data(), C * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.cl, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    std::vector<int> cslot(N);
-    for (int cc = 0; cc < C; ++cc)
-        for (int q = 0; q < cn[cc]; ++q) cslot[cl[(size_t)cc * kCellCap + q]] = cc * kCellCap + q;
     HIPCHK(c, hipMemcpyAsync(D.cslot, cslot.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-
-    // groups and sweep levels
-    std::vector<int> group_of;
-    greedy_groups(hc, c->group_size, c->group_cells, group_of);
-    c->ngroups = (int)c->group_size.size();
-    int all26[26];
-    for (int k = 0; k < 26; ++k) all26[k] = k;
-    const int faces[6] = {0, 13, 2, 14, 12, 15};
-    make_levels(hc, c->group_size, c->group_cells, group_of, all26, 26, false, c->spin_fwd);
-    make_levels(hc, c->group_size, c->group_cells, group_of, all26, 26, true, c->spin_bwd);
-    make_levels(hc, c->group_size, c->group_cells, group_of, faces, 6, false, c->dte_fwd);
-    make_levels(hc, c->group_size, c->group_cells, group_of, faces, 6, true, c->dte_bwd);
+    merge_levels(sf, c->reps, c->spin_fwd);
+    merge_levels(sb, c->reps, c->spin_bwd);
+    merge_levels(tf, c->reps, c->dte_fwd);
+    merge_levels(tb, c->reps, c->dte_bwd);
     TRY(upload_levels(c, c->spin_fwd));
     TRY(upload_levels(c, c->spin_bwd));
     TRY(upload_levels(c, c->dte_fwd));
     TRY(upload_levels(c, c->dte_bwd));
 
     // electron temperatures (map_cells.cpp:128-159)
-    if (P.eltemp) {
-        std::vector<double> te(C, P.temperature), ee(C);
+    if (P0.eltemp) {
+        std::vector<double> te(C, P0.temperature), ee(C);
         const double bb = 2e-4 / kBoltz;
         for (int k = 0; k < C; ++k) ee[k] = 3e0 / bb * std::log(std::cosh(bb * te[k]));
-        TRY(up1(c, D.Te, te.data(), C));
-        TRY(up1(c, D.Ee, ee.data(), C));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        TRY(up1(c, D.Te, te.data(), 0, C));
+        TRY(up1(c, D.Ee, ee.data(), 0, C));
     }
     // initial.cpp:78-87
     TRY(nbuild(c, compute_forces ? 1 : 0));
     if (compute_forces) TRY(do_force(c));
-    if (P.eltemp) TRY(do_temperature(c));
-    c->total_time = P.start_time;
+    if (P0.eltemp) TRY(do_temperature(c));
+    c->total_time = P0.start_time;
     c->pass = 0;
     TRY(do_check(c, -1, c->d_obs));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -776,12 +892,25 @@ int sld_device_info(char* name, int len, int* n_cu) {
 }
 
 int sld_create(const sld_params* params, const sld_atoms* init, int compute_initial_forces, sld_ctx** out) {
-    if (!params || !out) return SLD_ERR_ARG;
+    return sld_create_batch(1, params, init, compute_initial_forces, out);
+}
+
+int sld_create_batch(int nrep, const sld_params* params, const sld_atoms* init, int compute_initial_forces,
+                     sld_ctx** out) {
+    if (!params || !init || !out) return SLD_ERR_ARG;
     *out = nullptr;
     sld_ctx* c = new sld_ctx();
-    int rc = create_impl(c, params, init, compute_initial_forces);
+    int rc = create_impl(c, nrep, params, init, compute_initial_forces);
     *out = c;  // returned even on failure so that sld_last_error can be read
     return rc;
+}
+
+int sld_nreplicas(const sld_ctx* c) { return c ? c->R : 0; }
+
+int sld_select_replica(sld_ctx* c, int r) {
+    if (r < 0 || r >= c->R) return set_err(c, SLD_ERR_ARG, "replica %d out of range (%d replicas)", r, c->R);
+    c->sel = r;
+    return SLD_OK;
 }
 
 int sld_destroy(sld_ctx* c) {
@@ -817,7 +946,7 @@ int sld_calculate_temperature(sld_ctx* c) {
 
 int sld_check(sld_ctx* c, int step, sld_obs* out) {
     TRY(do_check(c, step, c->d_obs));
-    if (out) HIPCHK(c, hipMemcpyAsync(out, c->d_obs, sizeof(sld_obs), hipMemcpyDeviceToHost, c->stream));
+    if (out) HIPCHK(c, hipMemcpyAsync(out, c->d_obs, sizeof(sld_obs) * c->R, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return read_flags(c);
 }
@@ -830,7 +959,7 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
         if ((s + 1) % every == 0) ++nprint;
     if (nprint > c->obs_cap) {
         sld_obs* nb = nullptr;
-        HIPCHK(c, hipMalloc(&nb, sizeof(sld_obs) * nprint));
+        HIPCHK(c, hipMalloc(&nb, sizeof(sld_obs) * nprint * c->R));
         c->allocs.push_back(nb);
         c->d_obs = nb;
         c->obs_cap = nprint;
@@ -845,7 +974,7 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
         TRY(do_core(c));
         c->total_time += c->prm.step;
         if ((s + 1) % every == 0) {
-            TRY(do_check(c, s, c->d_obs + k));
+            TRY(do_check(c, s, c->d_obs + (size_t)k * c->R));
             ++k;
         }
     }
@@ -866,7 +995,7 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
         c->timed_launches += c->ev_launches[k / 2];
     }
     if (obs && k > 0)
-        HIPCHK(c, hipMemcpy(obs, c->d_obs, sizeof(sld_obs) * k, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(obs, c->d_obs, sizeof(sld_obs) * k * c->R, hipMemcpyDeviceToHost));
     if (nobs) *nobs = k;
     return read_flags(c);
 }
@@ -891,17 +1020,20 @@ int sld_kernel_timing(sld_ctx* c, int* launches, double* total_ms) {
 int sld_download(sld_ctx* c, sld_atoms* o) {
     if (!o) return SLD_ERR_ARG;
     Dev& D = c->D;
-    const int N = D.N;
-    TRY(down4(c, D.r, o->r)); TRY(down4(c, D.p, o->p)); TRY(down4(c, D.f, o->f));
-    TRY(down4(c, D.s, o->s)); TRY(down4(c, D.hh, o->heff_h)); TRY(down4(c, D.hl, o->heff_l));
-    TRY(down1(c, D.rho, o->rho, N)); TRY(down1(c, D.pe, o->pe, N)); TRY(down1(c, D.ke, o->ke, N));
-    TRY(down1(c, D.me, o->me, N)); TRY(down1(c, D.me0, o->me0, N)); TRY(down1(c, D.vir, o->vir, N));
-    TRY(down1(c, D.sjs, o->sum_jij_sj, N));
+    const int r = c->sel;
+    const Rep& R = c->reps[r];
+    const size_t a0 = R.atom0, n = R.natom;
+    TRY(down4(c, D.r, o->r, r)); TRY(down4(c, D.p, o->p, r)); TRY(down4(c, D.f, o->f, r));
+    TRY(down4(c, D.s, o->s, r)); TRY(down4(c, D.hh, o->heff_h, r)); TRY(down4(c, D.hl, o->heff_l, r));
+    TRY(down1(c, D.rho, o->rho, a0, n)); TRY(down1(c, D.pe, o->pe, a0, n)); TRY(down1(c, D.ke, o->ke, a0, n));
+    TRY(down1(c, D.me, o->me, a0, n)); TRY(down1(c, D.me0, o->me0, a0, n)); TRY(down1(c, D.vir, o->vir, a0, n));
+    TRY(down1(c, D.sjs, o->sum_jij_sj, a0, n));
     if (o->stress) {
-        std::vector<double> st((size_t)6 * N);
-        TRY(down1(c, D.stress, st.data(), st.size()));
-        for (int i = 0; i < N; ++i)
-            for (int k = 0; k < 6; ++k) o->stress[6 * i + k] = st[(size_t)k * N + i];
+        std::vector<double> st(n);
+        for (int k = 0; k < 6; ++k) {
+            TRY(down1(c, D.stress, st.data(), (size_t)k * D.N + a0, n));
+            for (size_t i = 0; i < n; ++i) o->stress[6 * i + k] = st[i];
+        }
     }
     return SLD_OK;
 }
@@ -909,12 +1041,13 @@ int sld_download(sld_ctx* c, sld_atoms* o) {
 int sld_upload(sld_ctx* c, const sld_atoms* in) {
     if (!in) return SLD_ERR_ARG;
     Dev& D = c->D;
-    TRY(up4(c, D.r, in->r, false));
-    TRY(up4(c, D.p, in->p, false));
-    TRY(up4(c, D.s, in->s, true));
-    TRY(up4(c, D.f, in->f, false));
-    TRY(up1(c, D.rho, in->rho, D.N));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int r = c->sel;
+    const Rep& R = c->reps[r];
+    TRY(up4(c, D.r, in->r, r, false));
+    TRY(up4(c, D.p, in->p, r, false));
+    TRY(up4(c, D.s, in->s, r, true));
+    TRY(up4(c, D.f, in->f, r, false));
+    TRY(up1(c, D.rho, in->rho, R.atom0, R.natom));
     if (in->r) c->nb_dirty = true;
     return SLD_OK;
 }
@@ -922,40 +1055,45 @@ int sld_upload(sld_ctx* c, const sld_atoms* in) {
 int sld_get_cells(sld_ctx* c, sld_cells* o) {
     if (!o) return SLD_ERR_ARG;
     Dev& D = c->D;
-    const int C = D.C;
-    TRY(down1(c, D.Te, o->te, C)); TRY(down1(c, D.Ee, o->ee, C)); TRY(down1(c, D.Tl, o->tl, C));
-    TRY(down1(c, D.TsR, o->ts_r, C)); TRY(down1(c, D.TsL, o->ts_l, C));
-    TRY(down1(c, D.Gel, o->gel, C)); TRY(down1(c, D.Ges, o->ges, C));
-    TRY(down1(c, D.sRup, o->sum_r_up, C)); TRY(down1(c, D.sRdn, o->sum_r_dn, C));
-    TRY(down1(c, D.sLup, o->sum_l_up, C)); TRY(down1(c, D.sLdn, o->sum_l_dn, C));
+    const Rep& R = c->reps[c->sel];
+    const size_t c0 = R.cell0, n = R.ncell;
+    TRY(down1(c, D.Te, o->te, c0, n)); TRY(down1(c, D.Ee, o->ee, c0, n)); TRY(down1(c, D.Tl, o->tl, c0, n));
+    TRY(down1(c, D.TsR, o->ts_r, c0, n)); TRY(down1(c, D.TsL, o->ts_l, c0, n));
+    TRY(down1(c, D.Gel, o->gel, c0, n)); TRY(down1(c, D.Ges, o->ges, c0, n));
+    TRY(down1(c, D.sRup, o->sum_r_up, c0, n)); TRY(down1(c, D.sRdn, o->sum_r_dn, c0, n));
+    TRY(down1(c, D.sLup, o->sum_l_up, c0, n)); TRY(down1(c, D.sLdn, o->sum_l_dn, c0, n));
     return SLD_OK;
 }
 
 int sld_set_cells_te(sld_ctx* c, const double* te) {
-    TRY(up1(c, c->D.Te, te, c->D.C));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Rep& R = c->reps[c->sel];
+    TRY(up1(c, c->D.Te, te, R.cell0, R.ncell));
     return SLD_OK;
 }
 
 int sld_get_geometry(sld_ctx* c, int* ncell3, int* ngroups, int* group_sizes) {
-    if (ncell3) { ncell3[0] = c->box.nx; ncell3[1] = c->box.ny; ncell3[2] = c->box.nz; }
-    if (ngroups) *ngroups = c->ngroups;
-    if (group_sizes) std::copy(c->group_size.begin(), c->group_size.end(), group_sizes);
+    const ReplicaHost& h = c->rh[c->sel];
+    if (ncell3) { ncell3[0] = h.hc.nx; ncell3[1] = h.hc.ny; ncell3[2] = h.hc.nz; }
+    if (ngroups) *ngroups = (int)h.group_size.size();
+    if (group_sizes) std::copy(h.group_size.begin(), h.group_size.end(), group_sizes);
     return SLD_OK;
 }
 
 int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
-    const int C = c->D.C;
+    const Rep& R = c->reps[c->sel];
+    const int C = R.ncell;
     std::vector<int> cn(C), cl((size_t)C * kCellCap);
-    HIPCHK(c, hipMemcpyAsync(cn.data(), c->D.cn, C * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(cl.data(), c->D.cl, cl.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cn.data(), c->D.cn + R.cell0, C * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cl.data(), c->D.cl + (size_t)R.cell0 * kCellCap, cl.size() * sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int mx = 0;
     for (int k = 0; k < C; ++k) {
         if (counts) counts[k] = cn[k];
         mx = std::max(mx, cn[k]);
         if (lists)
-            for (int q = 0; q < std::min(cn[k], cap); ++q) lists[(size_t)k * cap + q] = cl[(size_t)k * kCellCap + q];
+            for (int q = 0; q < std::min(cn[k], cap); ++q)
+                lists[(size_t)k * cap + q] = cl[(size_t)k * kCellCap + q] - R.atom0;
     }
     if (mx > cap) return set_err(c, SLD_ERR_CAPACITY, "cell holds %d atoms > cap %d", mx, cap);
     return SLD_OK;

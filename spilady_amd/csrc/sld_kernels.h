@@ -8,10 +8,29 @@
 
 namespace sld {
 
-// Per-atom arrays are indexed by the caller's atom index; vectors are double4
-// with .w used as a payload (s.w = |s|, others 0).
+// One independent system of a batch: its geometry, thermostat temperature
+// and noise stream, and where its atoms and link cells sit in the
+// concatenated arrays. A replica's atom slots start on a 256-slot boundary and
+// its cells on an 8-cell boundary, so that every block of the per-atom and
+// per-cell reductions belongs to one replica; slots and cells in the padding
+// have replica -1.
+struct Rep {
+    Box box;
+    double temperature;   // k_B T, eV
+    double box_volume;
+    uint64_t seed;
+    int replica;          // noise stream id
+    int natom, atom0;     // atoms [atom0, atom0 + natom)
+    int ncell, cell0;     // link cells [cell0, cell0 + ncell)
+};
+
+// Per-atom arrays are indexed by atom slot (replica atom0 + the caller's atom
+// index); vectors are double4 with .w used as a payload (s.w = |s|, others 0).
 struct Dev {
-    int N, C;
+    int N, C;                  // atom slots and cells over all replicas, padding included
+    const Rep* rep;            // [R]
+    const int* arep;           // replica of each atom slot, -1 for padding [N]
+    const int* crep;           // replica of each cell, -1 for padding [C]
     double4 *r, *p, *f, *s, *hh, *hl;
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
     double* stress;            // [6][N]: 11 22 33 12 23 31
@@ -28,27 +47,28 @@ struct Dev {
     int* err;                  // error flag bits
 };
 
+// Parameters shared by every replica of a batch.
 struct Phys {
-    double atmass, gamma_L, gamma_S, temperature;
+    double atmass, gamma_L, gamma_S;
     double rcut_pot, rcut_mag, rcut_pot_sq, rcut_mag_sq, rcut_max_sq;
-    double box_volume;
-    uint64_t seed;
-    int replica;
     int spinlang, lattlang, eltemp, localcolmot;
 };
 
+// per-replica block of reduced scalars
+constexpr int kScal = 64;
+
 // ---- kernels ----
-__global__ void k_bin(Box b, Dev D);
-__global__ void k_relink(Box b, const int* cl_old, const int* cn_old, Dev D);
-__global__ void k_nbuild(Box b, Tables T, Phys P, Dev D, int write_rho);
+__global__ void k_bin(Dev D);
+__global__ void k_relink(const int* cl_old, const int* cn_old, Dev D);
+__global__ void k_nbuild(Tables T, Phys P, Dev D, int write_rho);
 __global__ void k_fp(Tables T, Dev D);
-__global__ void k_force(Box b, Tables T, Phys P, Dev D);
+__global__ void k_force(Tables T, Phys P, Dev D);
 __global__ void k_spin_phase(const int* cells, int ncell, int backward, double dt, Tables T, Phys P,
                              Dev D, uint64_t pass);
 __global__ void k_dp_cells(double dt, Phys P, Dev D, uint64_t pass, double* partials);
-__global__ void k_dp_center(Phys P, Dev D, const double* mean3, double* partials);
-__global__ void k_dp_scale(Phys P, Dev D, const double* ke2);
-__global__ void k_dr(double dt, Box b, Phys P, Dev D);
+__global__ void k_dp_center(Phys P, Dev D, const double* scal, double* partials);
+__global__ void k_dp_scale(Phys P, Dev D, const double* scal);
+__global__ void k_dr(double dt, Phys P, Dev D);
 __global__ void k_temperature(Tables T, Phys P, Dev D);
 __global__ void k_dte_ab(double h, Phys P, Dev D, double* partials);
 __global__ void k_dte_sweep(const int* cells, int ncell, double pref, double dx2, double dy2,
@@ -60,6 +80,7 @@ __global__ void k_obs_atoms(Tables T, Phys P, Dev D, double* partials, int scan_
 __global__ void k_obs_cells(Dev D, double* partials);
 __global__ void k_renorm(Dev D, const double* err, double* partials);
 __global__ void k_reduce(const double* partials, int nblk, int nk, double* out);
+__global__ void k_reduce_seg(const double* partials, const int* seg, int nk, double* out, int out_off);
 __global__ void k_check_nonfinite(Dev D);
 
 constexpr int kObsAtomK = 11;  // pe ke me me0 sx sy sz rup rdn lup ldn

@@ -54,6 +54,27 @@ __global__ __launch_bounds__(256) void k_reduce(const double* partials, int nblk
     }
 }
 
+// Per-replica sums: block r of the grid reduces partial rows [seg[r], seg[r+1])
+// in the same order as k_reduce, into out[r * kScal + out_off + k].
+__global__ __launch_bounds__(256) void k_reduce_seg(const double* partials, const int* seg, int nk, double* out,
+                                                    int out_off) {
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    const int b0 = seg[blockIdx.x], nb = seg[blockIdx.x + 1] - b0;
+    for (int k = 0; k < nk; ++k) {
+        double acc = 0.0;
+        for (int b = t; b < nb; b += 256) acc += partials[(size_t)(b0 + b) * nk + k];
+        sh[t] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) sh[t] += sh[t + s];
+            __syncthreads();
+        }
+        if (t == 0) out[(size_t)blockIdx.x * kScal + out_off + k] = sh[0];
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // links (links_CPU.cpp:85-177)
 
@@ -65,27 +86,35 @@ __device__ __forceinline__ bool cells_adjacent(const Box& b, int c0, int c1) {
 }
 
 // new cell of every atom (clamped binning of links_CPU.cpp:96-112)
-__global__ __launch_bounds__(256) void k_bin(Box b, Dev D) {
+__global__ __launch_bounds__(256) void k_bin(Dev D) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= D.N) return;
-    int old = D.cell[a];
-    int c = bin_cell(b, d3(D.r[a]));
+    const int rr = D.arep[a];
+    if (rr < 0) return;
+    const Rep& R = D.rep[rr];
+    const int old = D.cell[a];
+    const int lc = bin_cell(R.box, d3(D.r[a]));
+    const int c = R.cell0 + lc;
     D.cell_prev[a] = old;
     D.cell[a] = c;
-    if (c != old && !cells_adjacent(b, old, c)) atomicOr(D.err, kErrFarMove);
+    if (c != old && !cells_adjacent(R.box, old - R.cell0, lc)) atomicOr(D.err, kErrFarMove);
 }
 
 // Rebuild cell c's list. The reference detaches every mover and pushes it at
 // the head of its new cell in ascending atom index (links_CPU.cpp:144-173),
 // so a cell's new list is [movers in, descending index] + [stayers, old order].
-__global__ __launch_bounds__(128) void k_relink(Box b, const int* cl_old, const int* cn_old, Dev D) {
+__global__ __launch_bounds__(128) void k_relink(const int* cl_old, const int* cn_old, Dev D) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.C) return;
+    const int rr = D.crep[c];
+    if (rr < 0) return;
+    const Rep& R = D.rep[rr];
+    const int lc = c - R.cell0;
     int mv[kCellCap];
     int nm = 0;
     bool over = false;
     for (int slot = 0; slot < 26; ++slot) {
-        int nc = neigh_cell(b, c, slot);
+        int nc = R.cell0 + neigh_cell(R.box, lc, slot);
         int n = cn_old[nc];
         for (int k = 0; k < n; ++k) {
             int a = cl_old[nc * kCellCap + k];
@@ -144,15 +173,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_nbuild(Box b, Tables T, Phys P, Dev D, int write_rho) {
+__global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int write_rho) {
     __shared__ int s_cand[kNbWaves][kMaxCand];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int c = blockIdx.x * kNbWaves + w;
-    const bool valid = c < D.C;
+    const int rr = __builtin_amdgcn_readfirstlane(c < D.C ? D.crep[c] : -1);
+    const bool valid = rr >= 0;
+    Box b{};
+    int cell0 = 0;
+    if (valid) {
+        b = D.rep[rr].box;
+        cell0 = D.rep[rr].cell0;
+    }
     int cid = 0, cnt = 0;
     if (valid && lane < 27) {
-        cid = lane < 26 ? neigh_cell(b, c, lane) : c;
+        cid = lane < 26 ? cell0 + neigh_cell(b, c - cell0, lane) : c;
         cnt = D.cn[cid];
     }
     int incl = cnt;
@@ -250,7 +286,7 @@ __global__ __launch_bounds__(256) void k_nbuild(Box b, Tables T, Phys P, Dev D, 
 // (calculate_force_energy_CPU.cpp:98-107)
 __global__ __launch_bounds__(256) void k_fp(Tables T, Dev D) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= D.N) return;
+    if (i >= D.N || D.arep[i] < 0) return;
     int n0;
     double ratio;
     interp_index(D.rho[i], T.in_over_rhomax, n0, ratio);
@@ -264,9 +300,13 @@ __global__ __launch_bounds__(256) void k_fp(Tables T, Dev D) {
     D.fp[i] = dF + (dA * s2 + dB * s4 + dC * s4 * s2 + dD * s4 * s4);
 }
 
-__global__ __launch_bounds__(256) void k_force(Box b, Tables T, Phys P, Dev D) {
+__global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.N) return;
+    const int rr = D.arep[i];
+    if (rr < 0) return;
+    const Rep& R = D.rep[rr];
+    const Box b = R.box;
     const double3 ri = d3(D.r[i]);
     const double4 si4 = D.s[i];
     const double3 si = d3(si4);
@@ -334,7 +374,7 @@ __global__ __launch_bounds__(256) void k_force(Box b, Tables T, Phys P, Dev D) {
     // here each atom keeps half of each of its pairs (same total virial)
     D.vir[i] = 0.5 * vir;
     // per-atom stress with the kinetic part (calculate_force_energy_CPU.cpp:283-293)
-    const double lv = P.box_volume / D.N;
+    const double lv = R.box_volume / R.natom;
     const double m = P.atmass;
     const double4 p = D.p[i];
     const size_t N = D.N;
@@ -481,7 +521,11 @@ __global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell,
     const int n = D.cn[c];
     if (n == 0) return;
     const int* lst = D.cl + c * kCellCap;
-    const double Tk = P.spinlang ? (P.eltemp ? D.Te[c] : P.temperature) : 0.0;
+    const Rep& R = D.rep[D.crep[c]];
+    const int atom0 = R.atom0;
+    const uint64_t seed = R.seed;
+    const int stream = R.replica;
+    const double Tk = P.spinlang ? (P.eltemp ? D.Te[c] : R.temperature) : 0.0;
     const double dth = dt / 2.0, dtq = dt / 4.0, gS = P.gamma_S;
     const double rs = P.spinlang ? sqrt(2.0 * Tk * gS / dt) : 0.0;
     auto atom_at = [&](int q) { return lst[backward ? n - 1 - q : q]; };
@@ -545,7 +589,7 @@ __global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell,
         if (!P.spinlang) {
             s = apply_rot(make_rot(H, dt), s);
         } else {
-            double3 z = normal3(P.seed, P.replica, a, pass);
+            double3 z = normal3(seed, stream, a - atom0, pass);
             int n0;
             double ratio;
             interp_index(rho, T.in_over_rhomax, n0, ratio);
@@ -619,7 +663,8 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, uint
                                                   double* partials) {
     const int l = threadIdx.x & 31;
     const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
-    const bool valid = c < D.C;
+    const int rr = c < D.C ? D.crep[c] : -1;
+    const bool valid = rr >= 0;
     const int n = valid ? D.cn[c] : 0;
     const bool has = l < n;
     const int a = has ? D.cl[c * kCellCap + l] : 0;
@@ -631,11 +676,12 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, uint
             if (P.localcolmot && n > 0) p = dp_colmot(p, l, n, gm * h / n, false);
             const double eb = P.localcolmot ? exp(-gm * h * (1.0 - 1.0 / n)) : exp(-gm * h);
             p = mul3(eb, p);
-            const double Tk = P.eltemp ? D.Te[c] : P.temperature;
+            const Rep& R = D.rep[rr];
+            const double Tk = P.eltemp ? D.Te[c] : R.temperature;
             const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
             double3 ff = make_double3(0.0, 0.0, 0.0);
             if (has) {
-                ff = mul3(len, normal3(P.seed, P.replica, a, pass));
+                ff = mul3(len, normal3(R.seed, R.replica, a - R.atom0, pass));
                 p = add3(p, mul3(dt, add3(d3(D.f[a]), ff)));
             }
             if (P.localcolmot && n > 0) {
@@ -654,12 +700,15 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, uint
     block_partials<3>(acc, partials);
 }
 
-// rescale_momentum, part 1: remove the mean momentum (core_dp_CPU.cpp:270-302)
-__global__ __launch_bounds__(256) void k_dp_center(Phys P, Dev D, const double* psum, double* partials) {
+// rescale_momentum, part 1: remove the mean momentum (core_dp_CPU.cpp:270-302);
+// the replica's momentum sum is scal[0..2] of its scalar block
+__global__ __launch_bounds__(256) void k_dp_center(Phys P, Dev D, const double* scal, double* partials) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[2] = {0.0, 0.0};
-    if (a < D.N) {
-        const double inv_n = 1.0 / D.N;
+    const int rr = a < D.N ? D.arep[a] : -1;
+    if (rr >= 0) {
+        const double* psum = scal + (size_t)rr * kScal;
+        const double inv_n = 1.0 / D.rep[rr].natom;
         double3 mean = make_double3(psum[0] * inv_n, psum[1] * inv_n, psum[2] * inv_n);
         double3 p = d3(D.p[a]);
         acc[0] = sq3(p) / 2.0 / P.atmass;
@@ -670,10 +719,14 @@ __global__ __launch_bounds__(256) void k_dp_center(Phys P, Dev D, const double* 
     block_partials<2>(acc, partials);
 }
 
-// rescale_momentum, part 2, and the kinetic energies (core_dp_CPU.cpp:78-84,304-310)
-__global__ __launch_bounds__(256) void k_dp_scale(Phys P, Dev D, const double* ke2) {
+// rescale_momentum, part 2, and the kinetic energies (core_dp_CPU.cpp:78-84,304-310);
+// the replica's kinetic energies before and after centring are scal[3..4]
+__global__ __launch_bounds__(256) void k_dp_scale(Phys P, Dev D, const double* scal) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= D.N) return;
+    const int rr = D.arep[a];
+    if (rr < 0) return;
+    const double* ke2 = scal + (size_t)rr * kScal + 3;
     double factor = sqrt(ke2[0] / ke2[1]);
     if (ke2[1] < 1e-10) factor = 1.0;
     double3 p = mul3(factor, d3(D.p[a]));
@@ -682,12 +735,14 @@ __global__ __launch_bounds__(256) void k_dp_scale(Phys P, Dev D, const double* k
 }
 
 // core_dr_CPU.cpp:33-44
-__global__ __launch_bounds__(256) void k_dr(double dt, Box b, Phys P, Dev D) {
+__global__ __launch_bounds__(256) void k_dr(double dt, Phys P, Dev D) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= D.N) return;
+    const int rr = D.arep[a];
+    if (rr < 0) return;
     double k = dt / P.atmass;
     double3 r = add3(d3(D.r[a]), mul3(k, d3(D.p[a])));
-    r = wrap(b, r);
+    r = wrap(D.rep[rr].box, r);
     D.r[a] = make_double4(r.x, r.y, r.z, 0.0);
 }
 
@@ -699,7 +754,7 @@ __global__ __launch_bounds__(256) void k_dr(double dt, Box b, Phys P, Dev D) {
 __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
     const int l = threadIdx.x & 31;
     const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
-    if (c >= D.C) return;
+    if (c >= D.C || D.crep[c] < 0) return;
     const int n = D.cn[c];
     const bool has = l < n;
     double ke = 0.0, rup = 0.0, rdn = 0.0, lup = 0.0, ldn = 0.0;
@@ -740,7 +795,8 @@ __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
     lup = seg_sum32(lup);
     ldn = seg_sum32(ldn);
     if (l != 0) return;
-    const double apc = (double)D.N / (double)D.C;
+    const Rep& R = D.rep[D.crep[c]];
+    const double apc = (double)R.natom / (double)R.ncell;
     const double gm = P.gamma_L / P.atmass;
     if (P.localcolmot) {
         ap = make_double3(ap.x / n, ap.y / n, ap.z / n);
@@ -920,7 +976,7 @@ __global__ __launch_bounds__(256) void k_obs_atoms(Tables T, Phys P, Dev D, doub
     double acc[kObsAtomK + 1];
 #pragma unroll
     for (int k = 0; k <= kObsAtomK; ++k) acc[k] = 0.0;
-    if (a < D.N) {
+    if (a < D.N && D.arep[a] >= 0) {
         acc[0] = D.pe[a];
         acc[1] = D.ke[a];
         acc[2] = D.me[a];

@@ -6,7 +6,9 @@
 // Built with -ffp-contract=off and the same libm calls as the reference so the
 // tables agree with the reference's entry for entry.
 #include <cmath>
+#include <mutex>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "sld_common.h"
@@ -186,6 +188,18 @@ void build_tables(double rcut_max, HostTables& t) {
         if (a < b) th.emplace_back(work, a, b);
     }
     for (auto& x : th) x.join();
+}
+
+std::shared_ptr<const HostTables> shared_tables(double rcut_max) {
+    static std::mutex mu;
+    static std::vector<std::pair<double, std::shared_ptr<const HostTables>>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& e : cache)
+        if (e.first == rcut_max) return e.second;
+    auto t = std::make_shared<HostTables>();
+    build_tables(rcut_max, *t);
+    cache.emplace_back(rcut_max, t);
+    return t;
 }
 
 }  // namespace sld

@@ -1,6 +1,7 @@
 // Host-side potential tables (see sld_tables.cpp).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 namespace sld {
@@ -12,5 +13,8 @@ struct HostTables {
 };
 
 void build_tables(double rcut_max, HostTables& t);
+// The tables for rcut_max, built once per process and shared by every
+// context created with the same cutoff.
+std::shared_ptr<const HostTables> shared_tables(double rcut_max);
 
 }  // namespace sld

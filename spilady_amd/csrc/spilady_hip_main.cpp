@@ -1,0 +1,256 @@
+// spilady_hip: the SPILADY driver (spilady.cpp:33-72) on libspilady_hip.so.
+//
+//   spilady_hip [variables.in]
+//
+// Reads the run parameters from variables.in (read_variables.cpp:40-254; the
+// control.h switches spinlang, lattlang, eltemp, localcolmot and
+// renormalizeEnergy are runtime keys here, 1 = on), the initial configuration
+// from the con- file named by in_config (read_config.cpp:32-147; the readconf
+// path: f and rho are taken from the file, initial.cpp:78-83), then runs
+// no_of_production_steps steps of core() (the runstep loop) with check() on
+// every printed step and write() on every configuration step, appending to
+// enr-/tmp-/spn-<out_body>.dat and writing con-/cel-<out_body>_NNNN.dat in
+// the reference's formats (check_energy_CPU.cpp:113-137,
+// check_temperature_CPU.cpp:136-161, check_spin_CPU.cpp:58-79,
+// write_config.cpp:31-206).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/spilady_hip.h"
+
+namespace {
+
+// spilady.h:53-62
+constexpr double kBoltz = 8.617343e-5;
+constexpr double kSpeedOfLight = 299792458e0;
+constexpr double kUtoEv = 931.494028e6 / kSpeedOfLight / kSpeedOfLight;
+constexpr double kLengthScale = 1e-10;
+
+[[noreturn]] void fail(const std::string& msg) {
+    std::fprintf(stderr, "spilady_hip: %s\n", msg.c_str());
+    std::exit(1);
+}
+
+// variables.in: whitespace-separated "key value" pairs, later keys win; the
+// defaults are the reference's (default.h:44-226)
+struct Vars {
+    std::map<std::string, std::string> kv = {
+        {"out_body", "spilady"}, {"no_of_production_steps", "1000"}, {"step", "1e-15"},
+        {"interval_of_print_out", "1"}, {"interval_of_config_out", "1000"}, {"in_config", "con-input.dat"},
+        {"kappa_e", "80"}, {"start_time", "0"}, {"atmass", "55.847"}, {"temperature", "300"},
+        {"gamma_L_over_mass", "6e11"}, {"gamma_S_HL", "5.88e13"}, {"random_seed", "1234"},
+        {"rcut_pot", "4.1"}, {"rcut_mag", "3.75"}, {"rcut_max", "4.1"}, {"min_length_link_cell", "4.11"},
+        {"spinlang", "1"}, {"lattlang", "1"}, {"eltemp", "1"}, {"localcolmot", "1"},
+        {"renormalizeEnergy", "1"}};
+    void read(const char* path) {
+        std::ifstream in(path);
+        if (!in) fail(std::string("cannot open ") + path);
+        std::string k, v;
+        while (in >> k >> v)
+            if (kv.count(k)) kv[k] = v;
+    }
+    double d(const char* k) const { return std::atof(kv.at(k).c_str()); }
+    long long i(const char* k) const { return (long long)std::atof(kv.at(k).c_str()); }
+    const std::string& s(const char* k) const { return kv.at(k); }
+};
+
+struct Atoms {
+    int n = 0;
+    double total_time = 0.0;
+    double box[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> r, p, f, s, hh, hl, stress, rho, pe, ke, me, me0, vir, sjs;
+    void resize(int natom) {
+        n = natom;
+        for (auto* v : {&r, &p, &f, &s, &hh, &hl}) v->assign((size_t)3 * n, 0.0);
+        for (auto* v : {&rho, &pe, &ke, &me, &me0, &vir, &sjs}) v->assign(n, 0.0);
+        stress.assign((size_t)6 * n, 0.0);
+    }
+    sld_atoms view() {
+        return {r.data(), p.data(), f.data(), s.data(), hh.data(), hl.data(), rho.data(), pe.data(),
+                ke.data(), me.data(), me0.data(), stress.data(), vir.data(), sjs.data()};
+    }
+};
+
+// con- file (read_config.cpp:39-108): "natom total_time TYPE", two box lines,
+// then "i element r p f stress[6] rho ke pe s s0 Heff_H Heff_L me me0 local_volume"
+Atoms read_con(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) fail("cannot open the initial configuration " + path);
+    Atoms a;
+    int n = 0;
+    std::string type;
+    in >> n >> a.total_time >> type;
+    if (type != "SLDHL") fail("the input configuration file is for " + type + ", not for SLDHL");
+    for (double& b : a.box) in >> b;
+    a.resize(n);
+    for (int i = 0; i < n; ++i) {
+        int idx;
+        std::string el;
+        double s0, lv, hx, hy, hz, lx, ly, lz;
+        in >> idx >> el;
+        for (int k = 0; k < 3; ++k) in >> a.r[3 * i + k];
+        for (int k = 0; k < 3; ++k) in >> a.p[3 * i + k];
+        for (int k = 0; k < 3; ++k) in >> a.f[3 * i + k];
+        for (int k = 0; k < 6; ++k) in >> a.stress[6 * i + k];
+        in >> a.rho[i] >> a.ke[i] >> a.pe[i];
+        for (int k = 0; k < 3; ++k) in >> a.s[3 * i + k];
+        in >> s0 >> hx >> hy >> hz >> lx >> ly >> lz >> a.me[i] >> a.me0[i] >> lv;
+        // the y columns repeat x (write_config.cpp:117,120); kept as read
+        a.hh[3 * i] = hx; a.hh[3 * i + 1] = hy; a.hh[3 * i + 2] = hz;
+        a.hl[3 * i] = lx; a.hl[3 * i + 1] = ly; a.hl[3 * i + 2] = lz;
+        if (!in) fail("con- file ends before atom " + std::to_string(i));
+    }
+    return a;
+}
+
+std::string numbered(const char* front, const std::string& body, int n_output) {
+    char buf[16];
+    std::snprintf(buf, sizeof(buf), "_%04d.dat", n_output);
+    return front + body + buf;
+}
+
+void append_logs(const std::string& body, const sld_obs& o, int eltemp) {
+    FILE* e = std::fopen(("enr-" + body + ".dat").c_str(), "a");
+    FILE* t = std::fopen(("tmp-" + body + ".dat").c_str(), "a");
+    FILE* s = std::fopen(("spn-" + body + ".dat").c_str(), "a");
+    if (!e || !t || !s) fail("cannot append to the enr-/tmp-/spn- files");
+    std::fprintf(e, "%d %.15e %.15e %.15e %.15e %.15e", o.step, o.total_time, o.ave_pe, o.ave_ke, o.ave_me,
+                 o.ave_me0);
+    if (eltemp) std::fprintf(e, " %.15e", o.ave_ee);
+    std::fprintf(e, "\n");
+    std::fprintf(t, "%d %.15e %.15e %.15e %.15e", o.step, o.total_time, o.tl, o.ts_r, o.ts_l);
+    if (eltemp) std::fprintf(t, " %.15e", o.te);
+    std::fprintf(t, "\n");
+    std::fprintf(s, "%d %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e\n", o.step, o.total_time,
+                 o.ave_s[0], o.ave_s[1], o.ave_s[2], o.ave_s0, o.ave_m[0], o.ave_m[1], o.ave_m[2], o.ave_m0);
+    std::fclose(e);
+    std::fclose(t);
+    std::fclose(s);
+}
+
+// write_config.cpp:31-206
+void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a, int eltemp) {
+    sld_atoms v = a.view();
+    if (sld_download(ctx, &v)) fail(sld_last_error(ctx));
+    const double t = sld_total_time(ctx);
+    FILE* f = std::fopen(numbered("con-", body, n_output).c_str(), "w");
+    if (!f) fail("cannot write the con- file");
+    std::fprintf(f, "%d %.15e SLDHL\n", a.n, t);
+    std::fprintf(f, "%.15e %.15e %.15e\n%.15e %.15e %.15e\n", a.box[0], a.box[1], a.box[2], a.box[3], a.box[4],
+                 a.box[5]);
+    for (int i = 0; i < a.n; ++i) {
+        const double* r = &a.r[3 * i];
+        const double* p = &a.p[3 * i];
+        const double* fo = &a.f[3 * i];
+        const double* st = &a.stress[6 * i];
+        const double* s = &a.s[3 * i];
+        const double* hh = &a.hh[3 * i];
+        const double* hl = &a.hl[3 * i];
+        const double s0 = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        std::fprintf(f,
+                     "%d Fe %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e "
+                     "%.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e "
+                     "%.15e %.15e\n",
+                     i, r[0], r[1], r[2], p[0], p[1], p[2], fo[0], fo[1], fo[2], st[0], st[1], st[2], st[3], st[4],
+                     st[5], a.rho[i], a.ke[i], a.pe[i], s[0], s[1], s[2], s0, hh[0], hh[0], hh[2], hl[0], hl[0],
+                     hl[2], a.me[i], a.me0[i], 0.0);
+    }
+    std::fclose(f);
+    if (!eltemp) return;
+    int n3[3], ng = 0;
+    if (sld_get_geometry(ctx, n3, &ng, nullptr)) fail(sld_last_error(ctx));
+    const int C = n3[0] * n3[1] * n3[2];
+    std::vector<double> te(C), tl(C), tsr(C), tsl(C);
+    sld_cells cv = {};
+    cv.te = te.data(); cv.tl = tl.data(); cv.ts_r = tsr.data(); cv.ts_l = tsl.data();
+    if (sld_get_cells(ctx, &cv)) fail(sld_last_error(ctx));
+    FILE* g = std::fopen(numbered("cel-", body, n_output).c_str(), "w");
+    if (!g) fail("cannot write the cel- file");
+    const double* d = a.box;
+    std::fprintf(g, "%d %.15e\n%.15e %.15e %.15e\n%.15e %.15e %.15e\n", C, t, d[0], d[1], d[2], d[3], d[4], d[5]);
+    for (int ix = 0; ix < n3[0]; ++ix)
+        for (int iy = 0; iy < n3[1]; ++iy)
+            for (int iz = 0; iz < n3[2]; ++iz) {
+                const int c = ix + iy * n3[0] + iz * n3[0] * n3[1];
+                const double qx = (ix + 0.5) / n3[0], qy = (iy + 0.5) / n3[1], qz = (iz + 0.5) / n3[2];
+                std::fprintf(g, "%.15e %.15e %.15e %.15e %.15e %.15e %.15e \n", d[0] * qx + d[1] * qy + d[3] * qz,
+                             d[2] * qy + d[4] * qz, d[5] * qz, te[c] / kBoltz, tl[c], tsr[c], tsl[c]);
+            }
+    std::fclose(g);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Vars v;
+    v.read(argc > 1 ? argv[1] : "variables.in");
+    Atoms a = read_con(v.s("in_config"));
+
+    // read_variables.cpp:155-159,258-276: input units to internal units
+    sld_params prm = {};
+    prm.natom = a.n;
+    prm.interval_of_print_out = (int)v.i("interval_of_print_out");
+    for (int k = 0; k < 6; ++k) prm.box[k] = a.box[k];
+    prm.step = v.d("step");
+    prm.temperature = v.d("temperature") * kBoltz;
+    prm.atmass = v.d("atmass") * kUtoEv * kLengthScale * kLengthScale;
+    prm.gamma_L_over_mass = v.d("gamma_L_over_mass");
+    prm.gamma_S_HL = v.d("gamma_S_HL");
+    prm.kappa_e = v.d("kappa_e");
+    prm.rcut_pot = v.d("rcut_pot");
+    prm.rcut_mag = v.d("rcut_mag");
+    prm.rcut_max = v.d("rcut_max");
+    prm.min_length_link_cell = v.d("min_length_link_cell");
+    if (prm.rcut_max > prm.min_length_link_cell) prm.min_length_link_cell = prm.rcut_max + 0.001;
+    prm.start_time = v.d("start_time");
+    prm.random_seed = (uint64_t)std::strtoull(v.s("random_seed").c_str(), nullptr, 10);
+    prm.spinlang = (int)v.i("spinlang");
+    prm.lattlang = (int)v.i("lattlang");
+    prm.eltemp = (int)v.i("eltemp");
+    prm.localcolmot = prm.eltemp ? (int)v.i("localcolmot") : 0;
+    prm.renormalize_energy = prm.eltemp ? (int)v.i("renormalizeEnergy") : 0;
+    const std::string body = v.s("out_body");
+    const long long nsteps = v.i("no_of_production_steps");
+    const int every_config = (int)std::max(1LL, v.i("interval_of_config_out"));
+    const int every_print = prm.interval_of_print_out > 0 ? prm.interval_of_print_out : 1;
+
+    std::printf("SPILADY (libspilady_hip) starts: %d atoms\n", a.n);
+    sld_ctx* ctx = nullptr;
+    sld_atoms init = a.view();
+    if (sld_create(&prm, &init, 0, &ctx)) fail(ctx ? sld_last_error(ctx) : "sld_create failed");
+
+    // check(-1) and write(-1) (spilady.cpp:38-40)
+    sld_obs o;
+    if (sld_check(ctx, -1, &o)) fail(sld_last_error(ctx));
+    append_logs(body, o, prm.eltemp);
+    write_config(ctx, body, 0, a, prm.eltemp);
+
+    // the runstep loop (spilady.cpp:43-59): advance to the next configuration
+    // output in one call; every printed step comes back as one record
+    std::vector<sld_obs> obs;
+    for (long long s = 0; s < nsteps;) {
+        long long next = (s / every_config + 1) * every_config;
+        if (next > nsteps) next = nsteps;
+        const int n = (int)(next - s);
+        obs.resize((size_t)n / every_print + 2);
+        int nobs = 0;
+        if (sld_step(ctx, (int)s, n, obs.data(), &nobs)) fail(sld_last_error(ctx));
+        for (int k = 0; k < nobs; ++k) {
+            append_logs(body, obs[k], prm.eltemp);
+            std::printf("%d total time = %.6e dt = %.3e\n", obs[k].step, obs[k].total_time, prm.step);
+        }
+        s = next;
+        if (s % every_config == 0) write_config(ctx, body, (int)((s - 1) / every_config + 1), a, prm.eltemp);
+    }
+    write_config(ctx, body, 9999, a, prm.eltemp);  // write(-2)
+    sld_destroy(ctx);
+    std::printf("SPILADY (libspilady_hip) ends\n");
+    return 0;
+}

@@ -56,6 +56,9 @@ def load(path: Optional[str] = None):
     L.sld_plan.argtypes = [_dp, C.c_double, _ip, _ip, _ip, _ip, _ip, _ip]
     L.sld_device_info.argtypes = [C.c_char_p, C.c_int, _ip]
     L.sld_create.argtypes = [P, A, C.c_int, C.POINTER(V)]
+    L.sld_create_batch.argtypes = [C.c_int, P, A, C.c_int, C.POINTER(V)]
+    L.sld_nreplicas.argtypes = [V]
+    L.sld_select_replica.argtypes = [V, C.c_int]
     L.sld_destroy.argtypes = [V]
     L.sld_last_error.argtypes = [V]
     L.sld_last_error.restype = C.c_char_p
@@ -87,18 +90,34 @@ class Engine(abi.EngineLike):
     context). Method names follow the reference entry points."""
 
     def __init__(self, params: abi.SldParams, atoms, compute_initial_forces: bool = False):
+        self._create([params], [atoms], compute_initial_forces)
+
+    def _create(self, plist, alist, compute_initial_forces):
         self.L = load()
-        self.params = params
-        self.natom = params.natom
-        self._in = abi.AtomBuffers.from_state(atoms)
+        self.nrep = len(plist)
+        self.params = plist[0]
+        self._natoms = [int(p.natom) for p in plist]
+        self._bufs = [abi.AtomBuffers.from_state(a) for a in alist]
+        self._pv = (abi.SldParams * self.nrep)(*plist)
+        self._av = (abi.SldAtoms * self.nrep)(*[b.view() for b in self._bufs])
         h = C.c_void_p()
-        rc = self.L.sld_create(C.byref(params), C.byref(self._in.view()),
-                               int(compute_initial_forces), C.byref(h))
+        if self.nrep == 1:
+            rc = self.L.sld_create(self._pv, self._av, int(compute_initial_forces), C.byref(h))
+        else:
+            rc = self.L.sld_create_batch(self.nrep, self._pv, self._av, int(compute_initial_forces),
+                                         C.byref(h))
         self.h = h
         if rc != 0:
             msg = self.L.sld_last_error(h).decode() if h else "sld_create failed"
             self.close()
             raise SldError(rc, msg)
+        self.select(0)
+
+    def select(self, replica: int):
+        """Replica that download/upload/cells/groups/cell_lists act on."""
+        self._rc(self.L.sld_select_replica(self.h, replica))
+        self.sel = replica
+        self.natom = self._natoms[replica]
         n3 = np.zeros(3, dtype=np.int32)
         ng = np.zeros(1, dtype=np.int32)
         self._rc(self.L.sld_get_geometry(self.h, n3.ctypes.data_as(_ip), ng.ctypes.data_as(_ip), None))
@@ -131,16 +150,23 @@ class Engine(abi.EngineLike):
     def core_dr(self, dt): self._rc(self.L.sld_core_dr(self.h, dt))
     def core_dTe(self, dt): self._rc(self.L.sld_core_dTe(self.h, dt))
 
-    def check(self, step: int) -> Dict[str, float]:
-        ob = abi.SldObs()
-        self._rc(self.L.sld_check(self.h, step, C.byref(ob)))
-        return abi.obs_dict(ob)
+    def _check_all(self, step: int):
+        ob = (abi.SldObs * self.nrep)()
+        self._rc(self.L.sld_check(self.h, step, ob))
+        return [abi.obs_dict(ob[r]) for r in range(self.nrep)]
 
-    def step(self, first_step: int, nsteps: int):
-        obs = (abi.SldObs * max(nsteps, 1))()
+    def _step_all(self, first_step: int, nsteps: int):
+        obs = (abi.SldObs * (max(nsteps, 1) * self.nrep))()
         k = C.c_int(0)
         self._rc(self.L.sld_step(self.h, first_step, nsteps, obs, C.byref(k)))
-        return [abi.obs_dict(obs[i]) for i in range(k.value)]
+        return [[abi.obs_dict(obs[i * self.nrep + r]) for r in range(self.nrep)] for i in range(k.value)]
+
+    def check(self, step: int) -> Dict[str, float]:
+        return self._check_all(step)[0]
+
+    def step(self, first_step: int, nsteps: int):
+        """nsteps of core() + check(); one observables dict per printed step."""
+        return [x[0] for x in self._step_all(first_step, nsteps)]
 
     def last_step_ms(self) -> float:
         ms = C.c_double(0.0)
@@ -191,6 +217,28 @@ class Engine(abi.EngineLike):
     @property
     def total_time(self) -> float:
         return self.L.sld_total_time(self.h)
+
+
+class BatchEngine(Engine):
+    """Independent replicas of any size, temperature and seed advanced
+    together: every kernel launch covers all of them (sld_create_batch)."""
+
+    def __init__(self, params_list, atoms_list, compute_initial_forces: bool = False):
+        if len(params_list) != len(atoms_list) or not params_list:
+            raise ValueError("one params and one atoms entry per replica")
+        self._create(list(params_list), list(atoms_list), compute_initial_forces)
+
+    def check(self, step: int):
+        """Observables of every replica (list of dicts)."""
+        return self._check_all(step)
+
+    def step(self, first_step: int, nsteps: int):
+        """nsteps of core() + check(); per printed step a list of per-replica dicts."""
+        return self._step_all(first_step, nsteps)
+
+    def download_replica(self, replica: int):
+        self.select(replica)
+        return self.download()
 
 
 def plan(box, min_length_link_cell: float):

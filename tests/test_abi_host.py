@@ -82,3 +82,37 @@ def test_plan_levels_even_box_merge_face_sweep():
     pl = engine.plan([L, 0, L, 0, 0, L], 4.11)
     assert pl["ncell3"] == (6, 6, 6)
     assert list(pl["nlevels"]) == [8, 8, 4, 4]
+
+
+def _batch_args(params_list):
+    from spilady_amd import abi as A
+    n = len(params_list)
+    ats = [configs.make_bcc_fe(6, 300.0, 1) for _ in range(n)]
+    bufs = [A.AtomBuffers.from_state(a) for a in ats]
+    pv = (A.SldParams * n)(*params_list)
+    av = (A.SldAtoms * n)(*[b.view() for b in bufs])
+    return pv, av, bufs
+
+
+def test_batch_rejects_mismatched_or_eltemp_replicas():
+    """Argument checks of sld_create_batch that run before any device call:
+    replicas must share dt and the switches, and eltemp needs one replica."""
+    lib = engine.load()
+    at = configs.make_bcc_fe(6, 300.0, 1)
+    v = configs.fe_variables(300.0, 1, thermostats=True)
+    p0 = abi.params_from_variables(v, at.box, at.natom)
+    p1 = abi.params_from_variables(v, at.box, at.natom, replica=1)
+    p1.step = 2e-15
+    pv, av, bufs = _batch_args([p0, p1])
+    h = C.c_void_p()
+    rc = lib.sld_create_batch(2, pv, av, 0, C.byref(h))
+    assert rc == 1 and b"must equal replica 0" in lib.sld_last_error(h)
+    lib.sld_destroy(h)
+    ve = configs.fe_variables(300.0, 1, thermostats=True, eltemp=True)
+    q0 = abi.params_from_variables(ve, at.box, at.natom)
+    q1 = abi.params_from_variables(ve, at.box, at.natom, replica=1)
+    pv, av, bufs = _batch_args([q0, q1])
+    h = C.c_void_p()
+    rc = lib.sld_create_batch(2, pv, av, 0, C.byref(h))
+    assert rc == 1 and b"single-system" in lib.sld_last_error(h)
+    lib.sld_destroy(h)

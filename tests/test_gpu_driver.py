@@ -1,0 +1,67 @@
+"""The C++ host driver (spilady_amd/spilady_hip, spilady.cpp's loop): its
+enr-/tmp-/spn- logs and con- files against the in-process engine started
+from the same con- file."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import consistent_state, fe_params
+from spilady_amd import configs, io
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "spilady_amd",
+                      "spilady_hip")
+
+
+def _setup(tmp, eltemp):
+    v, p, at = fe_params(8, 600.0, 21, thermostats=True, eltemp=eltemp, localcolmot=eltemp)
+    st = consistent_state(p, at)
+    state = io.AtomState(box=np.asarray(at.box), r=st["r"], p=st["p"], f=st["f"], s=st["s"],
+                         stress=st["stress"], rho=st["rho"], ke=st["ke"], pe=st["pe"], me=st["me"],
+                         me0=st["me0"])
+    io.write_con(os.path.join(tmp, "con-start.dat"), state)
+    v = dict(v, in_config="con-start.dat", out_body="drv", no_of_production_steps=7,
+             interval_of_print_out=2, interval_of_config_out=3)
+    io.write_variables(os.path.join(tmp, "variables.in"), v)
+    return v
+
+
+@pytest.mark.parametrize("eltemp", [False, True])
+def test_driver_logs_match_engine(tmp_path, eltemp):
+    from spilady_amd import abi
+    from spilady_amd.engine import Engine
+    tmp = str(tmp_path)
+    v = _setup(tmp, eltemp)
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    # the same start, read back from the file the driver read
+    st = io.read_con(os.path.join(tmp, "con-start.dat"))
+    p = abi.params_from_variables(io.read_variables(os.path.join(tmp, "variables.in")), st.box, st.natom)
+    e = Engine(p, st, compute_initial_forces=False)
+    obs = [e.check(-1)] + e.step(0, 7)
+    fmt = lambda x: float("%.15e" % x)  # noqa: E731
+    enr = io.read_log(os.path.join(tmp, "enr-drv.dat"))
+    tmpl = io.read_log(os.path.join(tmp, "tmp-drv.dat"))
+    spn = io.read_log(os.path.join(tmp, "spn-drv.dat"))
+    assert enr.shape[0] == len(obs) == 4          # steps -1, 1, 3, 5
+    for row_e, row_t, row_s, o in zip(enr, tmpl, spn, obs):
+        assert int(row_e[0]) == o["step"]
+        want_e = [o["ave_pe"], o["ave_ke"], o["ave_me"], o["ave_me0"]] + ([o["ave_ee"]] if eltemp else [])
+        assert list(row_e[2:]) == [fmt(x) for x in want_e]
+        want_t = [o["tl"], o["ts_r"], o["ts_l"]] + ([o["te"]] if eltemp else [])
+        assert list(row_t[2:]) == [fmt(x) for x in want_t]
+        want_s = list(o["ave_s"]) + [o["ave_s0"]] + list(o["ave_m"]) + [o["ave_m0"]]
+        assert list(row_s[2:]) == [fmt(x) for x in want_s]
+    # configurations after steps 3 and 6, and the final one (write(-2))
+    names = sorted(f for f in os.listdir(tmp) if f.startswith("con-drv_"))
+    assert names == ["con-drv_0000.dat", "con-drv_0001.dat", "con-drv_0002.dat", "con-drv_9999.dat"]
+    fin = io.read_con(os.path.join(tmp, "con-drv_9999.dat"))
+    a = e.download()
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+        np.testing.assert_array_equal(getattr(fin, k), np.vectorize(fmt)(a[k]), err_msg=k)
+    if eltemp:
+        assert os.path.exists(os.path.join(tmp, "cel-drv_9999.dat"))
+    e.close()

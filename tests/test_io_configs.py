@@ -72,3 +72,16 @@ def test_min_length_link_cell_raised_to_rcut():
     v["min_length_link_cell"] = 3.0
     p = abi.params_from_variables(v, [10, 0, 10, 0, 0, 10], 10)
     assert abs(p.min_length_link_cell - (4.1001 + 0.001)) < 1e-12
+
+
+def test_driver_reports_missing_inputs(tmp_path):
+    """The C++ driver fails with a message, before touching any device, when
+    variables.in or the initial configuration is missing."""
+    import subprocess
+    drv = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "spilady_amd",
+                       "spilady_hip")
+    r = subprocess.run([drv, "nope.in"], cwd=tmp_path, capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1 and "cannot open nope.in" in r.stderr
+    (tmp_path / "variables.in").write_text("in_config con-missing.dat\n")
+    r = subprocess.run([drv], cwd=tmp_path, capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1 and "con-missing.dat" in r.stderr
