@@ -95,6 +95,33 @@ def build_workload(n_unit: int):
     return v, p, at
 
 
+def nve_drift(nsteps: int, every: int = 100):
+    """Energy conservation of the microcanonical step: 50^3 cubic cells =
+    250,000 atoms, 300 K, seed 1, thermostats off, dt = 1 fs. E = pe + ke + me
+    per atom (the enr- columns) every `every` steps."""
+    from spilady_amd import abi, configs
+    from spilady_amd.engine import Engine
+    at = configs.make_bcc_fe(50, 300.0, 1)
+    v = configs.fe_variables(300.0, 1, thermostats=False, print_every=every)
+    p = abi.params_from_variables(v, at.box, at.natom)
+    eng = Engine(p, at, compute_initial_forces=True)
+    o0 = eng.check(-1)
+    t0 = time.perf_counter()
+    obs = eng.step(0, nsteps)
+    wall = time.perf_counter() - t0
+    eng.close()
+    e = np.array([o0["ave_pe"] + o0["ave_ke"] + o0["ave_me"]] +
+                 [o["ave_pe"] + o["ave_ke"] + o["ave_me"] for o in obs])
+    t = np.array([0.0] + [o["total_time"] for o in obs]) * 1e12     # ps
+    slope = float(np.polyfit(t, e, 1)[0])
+    return {"metric": "NVE energy drift, 250k-atom bcc Fe SLD (FP64)", "steps": nsteps,
+            "dt_fs": p.step * 1e15, "e0_ev_per_atom": float(e[0]), "e_end_ev_per_atom": float(e[-1]),
+            "max_abs_dev_ev_per_atom": float(np.max(np.abs(e - e[0]))),
+            "drift_ev_per_atom_per_ps": slope, "ke_ev_per_atom": float(obs[-1]["ave_ke"]),
+            "ms_per_step": wall / nsteps * 1e3, "atom_steps_per_s": at.natom * nsteps / wall,
+            "series_every": every, "series": e.tolist()}
+
+
 def batch_workload(nrep: int = 64, seed: int = 3):
     """64 independent bcc Fe replicas: L^3 cubic cells with L drawn uniformly
     from {8,12,...,32}, temperatures evenly spaced over 100-1400 K, seed
@@ -187,6 +214,8 @@ def main() -> int:
                     help="kernels timed with HIP events in extra steps after the timed region")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nve-drift", type=int, default=0,
+                    help="instead of the benchmark, measure NVE energy drift over this many steps")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="timed steps of the batched-vs-serial replica comparison (0: skip)")
     args = ap.parse_args()
@@ -194,6 +223,9 @@ def main() -> int:
         print("bench.py runs on one GPU: the spin-lattice engine is single-device in this build "
               "(see DESIGN.md for how the path would shard).", file=sys.stderr)
         return 2
+    if args.nve_drift > 0:
+        print(json.dumps(nve_drift(args.nve_drift)), flush=True)
+        return 0
 
     from spilady_amd.engine import Engine, device_info
     v, p, at = build_workload(args.n_unit)

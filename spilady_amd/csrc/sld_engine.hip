@@ -60,6 +60,11 @@ struct sld_ctx {
     Tables tab{};
     Dev D{};
     hipStream_t stream = nullptr;
+    // core_dTe(dt) issues the same launches on every call with the same dt: it
+    // is captured once into a graph and replayed
+    bool use_graphs = true;
+    hipGraphExec_t dte_exec = nullptr;
+    double dte_exec_dt = 0.0;
     int device = 0;
     LevelSet spin_fwd, spin_bwd, dte_fwd, dte_bwd;  // over all replicas
     // buffers
@@ -345,7 +350,36 @@ int do_temperature(sld_ctx* c) {
 }
 
 // core_dTe_CPU.cpp:47-217 with nloop = 50
+int do_dTe_launches(sld_ctx* c, double dt);
+
+// core_dTe_CPU.cpp:47-217 through a graph of its launches when graphs are on
+// and no kernel is being timed
 int do_dTe(sld_ctx* c, double dt) {
+    if (!c->use_graphs || !c->timed_name.empty()) return do_dTe_launches(c, dt);
+    if (c->dte_exec && c->dte_exec_dt != dt) {
+        (void)hipGraphExecDestroy(c->dte_exec);
+        c->dte_exec = nullptr;
+    }
+    if (!c->dte_exec) {
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = do_dTe_launches(c, dt);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != SLD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_err(c, SLD_ERR_HIP, "capture of core_dTe: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->dte_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err(c, SLD_ERR_HIP, "instantiate core_dTe graph: %s", hipGetErrorString(e));
+        c->dte_exec_dt = dt;
+    }
+    HIPCHK(c, hipGraphLaunch(c->dte_exec, c->stream));
+    return SLD_OK;
+}
+
+int do_dTe_launches(sld_ctx* c, double dt) {
     const int nloop = 50;
     double dtl = dt / nloop;
     double h = dtl / 2.0;
@@ -625,6 +659,10 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
         return set_err(c, SLD_ERR_NOGPU, "no HIP device");
     HIPCHK(c, hipGetDevice(&c->device));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        const char* g = std::getenv("SLD_GRAPHS");
+        c->use_graphs = !(g && g[0] == '0');
+    }
 
     // geometry of every replica (read_config.cpp:110-116, vec_utils_CPU.cpp:135-148,
     // map_cells.cpp:41-60) and its place in the concatenated arrays
@@ -916,6 +954,7 @@ int sld_select_replica(sld_ctx* c, int r) {
 int sld_destroy(sld_ctx* c) {
     if (!c) return SLD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->dte_exec) (void)hipGraphExecDestroy(c->dte_exec);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);

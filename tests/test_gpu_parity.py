@@ -153,3 +153,28 @@ def test_core_dte_hot_spot(nu):
     # diffusion moved heat out of the hot spot, and the total is kept
     assert b.max() < te.max()
     eng.close(); orc.close()
+
+
+def test_core_dte_graph_replays_equal_eager_launches(monkeypatch):
+    """core_dTe is captured into a graph on its first call and replayed after:
+    four replays, with eager momentum, temperature and download work between
+    them, bitwise against an engine that launches every kernel (SLD_GRAPHS=0)."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(16, 600.0, 9, thermostats=True, eltemp=True, localcolmot=True, renormalize=False)
+    st = consistent_state(p, at)
+    engines = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SLD_GRAPHS", flag)
+        engines.append(Engine(p, st, compute_initial_forces=False))
+    dt = p.step
+    for k in range(4):
+        for e in engines:
+            e.core_dTe(dt / 2)
+            e.core_dp(dt / 2)
+            e.calculate_temperature()
+        a, b = (e.cells() for e in engines)
+        for f in ("te", "tl", "gel", "ts_l"):
+            np.testing.assert_array_equal(a[f], b[f], err_msg=f"replay {k} {f}")
+        np.testing.assert_array_equal(engines[0].download()["p"], engines[1].download()["p"])
+    for e in engines:
+        e.close()
