@@ -5,7 +5,9 @@ run of the same start (tests/golden/nve250k.npz, 8 OpenMP threads).
 Every step's energies, temperatures and mean spin, and r, p, f, s, H_eff of
 every 250th atom after steps 1, 10, 100, 250 and 500, must agree to 1e-10
 (normwise relative per column or array; the fixture holds the values printed
-with 16 significant digits)."""
+with 16 significant digits). The start is rebuilt from the same generator
+and seed with the oracle's initial forces, which equal the reference's to
+rounding."""
 import os
 
 import numpy as np
@@ -45,6 +47,10 @@ def test_nve_250k_500_steps_against_reference():
     for c, k in ((2, 0), (3, 1), (4, 2)):
         assert_close(g["spn"][:, c], np.array([o["ave_s"][k] for o in obs]), f"ave_s[{k}]", TOL)
     assert_close(g["spn"][:, 9], np.array([o["ave_m0"] for o in obs]), "ave_m0", TOL)
-    # total energy per atom, kinetic + potential + magnetic
-    etot = cols["ave_pe"] + cols["ave_ke"] + cols["ave_me"]
-    assert np.ptp(etot) < 2e-5 * abs(etot.mean())
+    # total energy per atom, kinetic + potential + magnetic: the reference's
+    # own value swings by about 9e-4 eV/atom over these 500 steps (the
+    # splitting error of the 1 fs step while the tilted spins precess); the
+    # GPU follows the same curve
+    etot_ref = cols["ave_pe"] + cols["ave_ke"] + cols["ave_me"]
+    etot = np.array([o["ave_pe"] + o["ave_ke"] + o["ave_me"] for o in obs])
+    assert np.max(np.abs(etot - etot_ref)) <= TOL * np.max(np.abs(etot_ref))
