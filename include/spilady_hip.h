@@ -187,6 +187,38 @@ int sld_last_step_timing(sld_ctx* ctx, double* total_ms);
 int sld_time_kernel(sld_ctx* ctx, const char* kernel);
 int sld_kernel_timing(sld_ctx* ctx, int* launches, double* total_ms);
 
+/* ---- Parametric potential model and its parameter adjoint ----
+ * No counterpart in the reference, whose potential (DD05.cpp, JijFe.cpp) is
+ * fixed at compile time; these are for callers that fit the potential's
+ * coefficients to energies, forces and effective fields. The configurations
+ * are the context's replicas, frozen at their current positions and spins.
+ * The parameter vector theta has sld_model_nparams() = 22 entries:
+ *   [0..5]   a_k of the density f(r) = sum a_k (R_k - r)^3          (DD05.cpp:20-43)
+ *   [6..13]  V_k of the pair term V(r) = sum V_k (R_k - r)^3, r > 2  (DD05.cpp:64-164;
+ *            the ZBL core below 1.8 and the quintic bridge follow)
+ *   [14..15] aa, bb of the embedding F(rho)                          (DD05.cpp:6-18)
+ *   [16..17] A, rc of the exchange J(r) = A (1 - r/rc)^3              (JijFe.cpp:5-14)
+ *   [18..21] Landau A, B, C, D                                        (JijFe.cpp:18-43)
+ * E_r = sum over replica r's atoms of pe + me + me0; F = -dE/dr and
+ * H = -dE/ds per atom. Atom arrays are compact over the replicas, replica
+ * r's atoms after those of replicas 0..r-1, each in its own index order
+ * (sld_model_natoms atoms in all). Buffers may be host or device memory. */
+int sld_model_nparams(void);
+/* The reference's values of theta. */
+int sld_model_default_params(double* theta);
+/* Builds the model's tables (10^6 samples each, as potential_CPU.cpp:61-170)
+ * at theta; about a second on the host. */
+int sld_model_set_params(sld_ctx* ctx, const double* theta);
+int sld_model_get_params(sld_ctx* ctx, double* theta);
+int sld_model_natoms(const sld_ctx* ctx);
+/* energy[nrep], force[3 * natoms], field[3 * natoms]; any may be NULL. */
+int sld_model_eval(sld_ctx* ctx, double* energy, double* force, double* field);
+/* grad[22] = sum_r g_energy[r] dE_r/dtheta + sum_i g_force_i . dF_i/dtheta
+ *          + sum_i g_field_i . dH_i/dtheta, exact for the tabulated model;
+ * NULL upstream gradients count as zero. */
+int sld_model_vjp(sld_ctx* ctx, const double* g_energy, const double* g_force, const double* g_field,
+                  double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,8 +28,9 @@ _ip = C.POINTER(C.c_int)
 def build(force: bool = False) -> str:
     """Compile oracle/sld_oracle.c (plain C, no FP contraction, so that the
     arithmetic order of the reference is kept)."""
-    src = os.path.join(HERE, "sld_oracle.c")
-    if force or not os.path.exists(ORC_LIB) or os.path.getmtime(ORC_LIB) < os.path.getmtime(src):
+    srcs = [os.path.join(HERE, f) for f in ("sld_oracle.c", "sld_model_oracle.c", "sld_oracle.h")]
+    if force or not os.path.exists(ORC_LIB) or any(os.path.getmtime(ORC_LIB) < os.path.getmtime(f)
+                                                   for f in srcs):
         subprocess.check_call(["make", "-s", "-C", HERE, "-f", "Makefile", "liborc.so"])
     return ORC_LIB
 
@@ -76,6 +77,23 @@ def lib():
         L.orc_analytic.restype = C.c_double
         L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normal3.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint64, _dp]
+        L.orc_pair_v.argtypes = [C.c_double, _dp]
+        L.orc_pair_v.restype = C.c_double
+        L.orc_bridge_coeffs.argtypes = [_dp, _dp]
+        L.orc_fields.argtypes = [C.c_void_p, _dp]
+        L.orcm_default_params.argtypes = [_dp]
+        L.orcm_gen.argtypes = [_dp, C.c_int, C.c_double]
+        L.orcm_gen.restype = C.c_double
+        L.orcm_dgen.argtypes = [_dp, C.c_int, C.c_int, C.c_double]
+        L.orcm_dgen.restype = C.c_double
+        L.orcm_build.argtypes = [_dp, C.c_double]
+        L.orcm_build.restype = C.c_void_p
+        L.orcm_free.argtypes = [C.c_void_p]
+        L.orcm_table_raw.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.orcm_table_raw.restype = C.c_double
+        L.orcm_eval_grad.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orcm_eval_grad.restype = C.c_int
         _lib = L
     return _lib
 
@@ -174,6 +192,12 @@ class Oracle(abi.EngineLike):
     def table_raw(self, which: int, n: int) -> float:
         return self.L.orc_table_raw(self.h, which, n)
 
+    def fields(self) -> np.ndarray:
+        """Effective field of every atom at the current spins (no update)."""
+        h = np.zeros((self.natom, 3))
+        self.L.orc_fields(self.h, h.ctypes.data_as(_dp))
+        return h
+
 
 def analytic(which: int, x: float) -> float:
     return lib().orc_analytic(which, x)
@@ -191,3 +215,89 @@ def normal3(seed: int, replica: int, atom: int, call: int):
     out = np.zeros(3)
     lib().orc_normal3(seed, replica, atom, call, out.ctypes.data_as(_dp))
     return out
+
+
+# ---------------------------------------------------------------------------
+# parametric potential model (oracle/sld_model_oracle.c)
+
+MODEL_EMBED, MODEL_DENS, MODEL_PAIR, MODEL_EXCH = 0, 1, 2, 3
+NPARAM = 22
+
+
+def _arr(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    if n is not None:
+        a = a.reshape(n)
+    return a
+
+
+def model_default_params() -> np.ndarray:
+    th = np.zeros(NPARAM)
+    lib().orcm_default_params(th.ctypes.data_as(_dp))
+    return th
+
+
+def model_gen(theta, kind: int, x: float) -> float:
+    th = _arr(theta, NPARAM)
+    return lib().orcm_gen(th.ctypes.data_as(_dp), kind, x)
+
+
+def model_dgen(theta, kind: int, k: int, x: float) -> float:
+    th = _arr(theta, NPARAM)
+    return lib().orcm_dgen(th.ctypes.data_as(_dp), kind, k, x)
+
+
+def pair_v(r: float, V) -> float:
+    v = _arr(V, 8)
+    return lib().orc_pair_v(r, v.ctypes.data_as(_dp))
+
+
+def bridge_coeffs(ends) -> np.ndarray:
+    e = _arr(ends, 6)
+    a = np.zeros(6)
+    lib().orc_bridge_coeffs(e.ctypes.data_as(_dp), a.ctypes.data_as(_dp))
+    return a
+
+
+class ModelTables:
+    """Tables of the model at theta and of their derivatives in theta."""
+
+    def __init__(self, theta, rcut_max: float):
+        self.theta = _arr(theta, NPARAM).copy()
+        self.L = lib()
+        self.h = self.L.orcm_build(self.theta.ctypes.data_as(_dp), float(rcut_max))
+
+    def raw(self, kind: int, deriv: int, n: int) -> float:
+        """kind >= 0: value table of that generator kind; kind = -1 - k: the
+        derivative table of parameter k."""
+        return self.L.orcm_table_raw(self.h, kind, deriv, n)
+
+    def eval_grad(self, atoms, rcut_max: float, rcut_pot: float, rcut_mag: float,
+                  gE: float = 0.0, gF=None, gH=None):
+        """(E, F[n,3], H[n,3], grad[22]) of one configuration (cubic box)."""
+        n = atoms.r.shape[0]
+        Lb = np.array([atoms.box[0], atoms.box[2], atoms.box[5]], dtype=np.float64)
+        r = _arr(atoms.r)
+        s = _arr(atoms.s)
+        F = np.zeros((n, 3))
+        H = np.zeros((n, 3))
+        E = np.zeros(1)
+        g = np.zeros(NPARAM)
+        pF = None if gF is None else _arr(gF, (n, 3))
+        pH = None if gH is None else _arr(gH, (n, 3))
+        rc = self.L.orcm_eval_grad(self.h, Lb.ctypes.data_as(_dp), n, r.ctypes.data_as(_dp),
+                                   s.ctypes.data_as(_dp), rcut_max, rcut_pot, rcut_mag, float(gE),
+                                   None if pF is None else pF.ctypes.data_as(_dp),
+                                   None if pH is None else pH.ctypes.data_as(_dp),
+                                   E.ctypes.data_as(_dp), F.ctypes.data_as(_dp), H.ctypes.data_as(_dp),
+                                   g.ctypes.data_as(_dp))
+        if rc != 0:
+            raise ValueError("box shorter than twice rcut_max")
+        return float(E[0]), F, H, g
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.orcm_free(self.h)
+            self.h = None
+
+    __del__ = close

@@ -113,33 +113,24 @@ static const double PV[8] = {-0.1960674387419232, 0.3687525935422963, -1.5053336
                              -1.792218099820337,  80.22069592246987};
 static const double PR[8] = {4.1, 3.8, 3.5, 3.2, 2.9, 2.6, 2.4, 2.3};
 
-/* DD05.cpp:64-164: ZBL below 1.8 A, quintic bridge to 2.0 A, cubic knots above */
-static double pair_gen(double rij) {
-    const int N = 8;
-    double pair = 0e0;
-    double x1 = 1.8, x2 = 2.0, offset = 0e0;
-    if (rij <= x1) {
-        pair = pairZBL(rij) + offset;
-    } else if (rij <= x2) {
-        double y1 = pairZBL(x1) + offset;
-        double Dy1 = (pairZBL(x1 + 1e-7) - pairZBL(x1 - 1e-7)) / 2e-7;
-        double D2y1 = (pairZBL(x1 + 1e-7) + pairZBL(x1 - 1e-7) - 2e0 * pairZBL(x1)) / pow(1e-7, 2);
-        double y2 = 0e0, Dy2 = 0e0, D2y2 = 0e0;
-        for (int i = 0; i < N; ++i) if (x2 < PR[i]) y2 += PV[i] * pow((PR[i] - x2), 3);
-        for (int i = 0; i < N; ++i) if (x2 < PR[i]) Dy2 -= 3e0 * PV[i] * pow((PR[i] - x2), 2);
-        for (int i = 0; i < N; ++i) if (x2 < PR[i]) D2y2 += 6e0 * PV[i] * (PR[i] - x2);
-        double a5 = (-6e0*Dy1*x1-6e0*Dy2*x1+D2y1*pow(x1,2)-D2y2*pow(x1,2)
+/* DD05.cpp:88-158: quintic through the ZBL end (x1 = 1.8) and the spline
+ * end (x2 = 2.0) matching value, first and second derivative at both ends;
+ * ends = {y1, Dy1, D2y1, y2, Dy2, D2y2}, a[k] the coefficient of r^k */
+void orc_bridge_coeffs(const double ends[6], double a[6]) {
+    double x1 = 1.8, x2 = 2.0;
+    double y1 = ends[0], Dy1 = ends[1], D2y1 = ends[2], y2 = ends[3], Dy2 = ends[4], D2y2 = ends[5];
+    a[5] = (-6e0*Dy1*x1-6e0*Dy2*x1+D2y1*pow(x1,2)-D2y2*pow(x1,2)
                    +6e0*Dy1*x2+6e0*Dy2*x2-2e0*D2y1*x1*x2+2e0*D2y2*x1*x2
                    +D2y1*pow(x2,2)-D2y2*pow(x2,2)+12*y1-12*y2)
                    /(2e0*pow((x1-x2),5));
-        double a4 = (-2e0*D2y1*pow(x1,3)+3e0*D2y2*pow(x1,3)+D2y1*pow(x1,2)*x2
+    a[4] = (-2e0*D2y1*pow(x1,3)+3e0*D2y2*pow(x1,3)+D2y1*pow(x1,2)*x2
                    -4e0*D2y2*pow(x1,2)*x2+4*D2y1*x1*pow(x2,2)-D2y2*x1*pow(x2,2)
                    -3e0*D2y1*pow(x2,3)+2e0*D2y2*pow(x2,3)
                    +2e0*Dy1*(7e0*pow(x1,2)+x1*x2-8e0*pow(x2,2))
                    +2e0*Dy2*(8e0*pow(x1,2)-x1*x2-7e0*pow(x2,2))
                    -30e0*x1*y1-30e0*x2*y1+30e0*x1*y2+30e0*x2*y2)
                    /(2e0*pow((x1-x2),5));
-        double a3 = -(-D2y1*pow(x1,4)+3e0*D2y2*pow(x1,4)-4e0*D2y1*pow(x1,3)*x2
+    a[3] = -(-D2y1*pow(x1,4)+3e0*D2y2*pow(x1,4)-4e0*D2y1*pow(x1,3)*x2
                    +8e0*D2y1*pow(x1,2)*pow(x2,2)-8e0*D2y2*pow(x1,2)*pow(x2,2)
                    +4e0*D2y2*x1*pow(x2,3)-3e0*D2y1*pow(x2,4)+D2y2*pow(x2,4)
                    +4e0*Dy1*(2e0*pow(x1,3)+8e0*pow(x1,2)*x2-7e0*x1*pow(x2,2)-3e0*pow(x2,3))
@@ -147,7 +138,7 @@ static double pair_gen(double rij) {
                    -20e0*pow(x1,2)*y1-80e0*x1*x2*y1-20e0*pow(x2,2)*y1
                    +20e0*pow(x1,2)*y2+80e0*x1*x2*y2+20e0*pow(x2,2)*y2)
                    /(2e0*pow((x1-x2),5));
-        double a2 = (D2y2*x1*pow((x1-x2),2)*(pow(x1,2)+6e0*x1*x2+3e0*pow(x2,2))
+    a[2] = (D2y2*x1*pow((x1-x2),2)*(pow(x1,2)+6e0*x1*x2+3e0*pow(x2,2))
                    -x2*(3e0*D2y1*pow(x1,4)-8e0*D2y1*pow(x1,2)*pow(x2,2)
                    +4e0*D2y1*x1*pow(x2,3)+D2y1*pow(x2,4)
                    -12e0*Dy1*x1*(2e0*pow(x1,2)+x1*x2-3e0*pow(x2,2))
@@ -155,25 +146,48 @@ static double pair_gen(double rij) {
                    +60e0*pow(x1,2)*y1+60e0*x1*x2*y1
                    -60e0*pow(x1,2)*y2-60e0*x1*x2*y2))
                    /(2e0*pow((x1-x2),5));
-        double a1 = (2e0*Dy2*pow(x1,2)*(pow(x1,3)-5e0*pow(x1,2)*x2-8e0*x1*pow(x2,2)+12e0*pow(x2,3))
+    a[1] = (2e0*Dy2*pow(x1,2)*(pow(x1,3)-5e0*pow(x1,2)*x2-8e0*x1*pow(x2,2)+12e0*pow(x2,3))
                   +x2*(-D2y2*pow(x1,2)*pow((x1-x2),2)*(2e0*x1+3e0*x2)
                   +x2*(-2e0*Dy1*(12e0*pow(x1,3)-8e0*pow(x1,2)*x2-5e0*x1*pow(x2,2)+pow(x2,3))
                   +x1*(D2y1*pow((x1-x2),2)*(3e0*x1+2e0*x2)+60e0*x1*(y1-y2)))))
                   /(2e0*pow((x1-x2),5));
-        double a0 = (D2y2*pow(x1,3)*pow((x1-x2),2)*pow(x2,2)+8e0*Dy1*pow(x1,3)*pow(x2,3)
+    a[0] = (D2y2*pow(x1,3)*pow((x1-x2),2)*pow(x2,2)+8e0*Dy1*pow(x1,3)*pow(x2,3)
                    -D2y1*pow(x1,4)*pow(x2,3)-10e0*Dy1*pow(x1,2)*pow(x2,4)
                    +2e0*D2y1*pow(x1,3)*pow(x2,4)+2e0*Dy1*x1*pow(x2,5)-D2y1*pow(x1,2)*pow(x2,5)
                    -2e0*Dy2*pow(x1,3)*x2*(pow(x1,2)-5e0*x1*x2+4e0*pow(x2,2))
                    -20e0*pow(x1,2)*pow(x2,3)*y1+10e0*x1*pow(x2,4)*y1-2e0*pow(x2,5)*y1
                    +2e0*pow(x1,5)*y2-10e0*pow(x1,4)*x2*y2+20e0*pow(x1,3)*pow(x2,2)*y2)
                    /(2e0*pow((x1-x2),5));
-        pair = a5*pow(rij,5)+a4*pow(rij,4)+a3*pow(rij,3)+a2*pow(rij,2)+a1*rij+a0;
+}
+
+/* DD05.cpp:64-164 with the knot coefficients V[8]: ZBL below 1.8 A, the
+ * quintic bridge to 2.0 A, cubic knots above */
+double orc_pair_v(double rij, const double V[8]) {
+    const int N = 8;
+    double pair = 0e0;
+    double x1 = 1.8, x2 = 2.0, offset = 0e0;
+    if (rij <= x1) {
+        pair = pairZBL(rij) + offset;
+    } else if (rij <= x2) {
+        double e[6], a[6];
+        e[0] = pairZBL(x1) + offset;
+        e[1] = (pairZBL(x1 + 1e-7) - pairZBL(x1 - 1e-7)) / 2e-7;
+        e[2] = (pairZBL(x1 + 1e-7) + pairZBL(x1 - 1e-7) - 2e0 * pairZBL(x1)) / pow(1e-7, 2);
+        double y2 = 0e0, Dy2 = 0e0, D2y2 = 0e0;
+        for (int i = 0; i < N; ++i) if (x2 < PR[i]) y2 += V[i] * pow((PR[i] - x2), 3);
+        for (int i = 0; i < N; ++i) if (x2 < PR[i]) Dy2 -= 3e0 * V[i] * pow((PR[i] - x2), 2);
+        for (int i = 0; i < N; ++i) if (x2 < PR[i]) D2y2 += 6e0 * V[i] * (PR[i] - x2);
+        e[3] = y2; e[4] = Dy2; e[5] = D2y2;
+        orc_bridge_coeffs(e, a);
+        pair = a[5]*pow(rij,5)+a[4]*pow(rij,4)+a[3]*pow(rij,3)+a[2]*pow(rij,2)+a[1]*rij+a[0];
     } else {
         for (int i = 0; i < N; ++i)
-            if (rij < PR[i]) pair += PV[i] * pow((PR[i] - rij), 3);
+            if (rij < PR[i]) pair += V[i] * pow((PR[i] - rij), 3);
     }
     return pair;
 }
+
+static double pair_gen(double rij) { return orc_pair_v(rij, PV); }
 
 static double Jij_gen(double rij) {
     double A = 0.870, rc = 3.75;
@@ -733,6 +747,23 @@ static inline vec landau_field(orc_ctx* o, int a, vec s, double A, double B, dou
     if (s0 > 0e0) HC = v_times(-o->sum_jij_sj[a] / s0, s);
     o->heff_hc[a] = HC;
     return v_add(L, HC);
+}
+
+/* Effective field at the current spins: inner_spin + inner_sum_Jij_sj +
+ * landau_field (calculate_spin_CPU.cpp:111-140,201-274) without the spin
+ * update; overwrites the heff_h, sum_jij_sj and heff_hc scratch of the atoms. */
+void orc_fields(orc_ctx* o, double* h) {
+    for (int a = 0; a < o->natom; ++a) {
+        o->heff_h[a] = v_zero();
+        inner_spin(o, a);
+        o->sum_jij_sj[a] = 0e0;
+        inner_sum_Jij_sj(o, a);
+        double ri = o->rho[a];
+        vec L = landau_field(o, a, o->s[a], TRHO(o, T_LA, ri), TRHO(o, T_LB, ri), TRHO(o, T_LC, ri),
+                             TRHO(o, T_LD, ri));
+        vec t = v_add(o->heff_h[a], L);
+        h[3 * a] = t.x; h[3 * a + 1] = t.y; h[3 * a + 2] = t.z;
+    }
 }
 
 /* calculate_spin_CPU.cpp:33-199 (SLDHL) */

@@ -58,6 +58,41 @@ double orc_table(const orc_ctx* o, int which, double x);
 double orc_table_raw(const orc_ctx* o, int which, int n);
 double orc_analytic(int which, double x);
 
+/* DD05 pair term with knot coefficients V[8] (DD05.cpp:64-164), and the
+ * coefficients a[k] (of r^k) of its quintic bridge from the end values
+ * {y1, y1', y1'', y2, y2', y2''} at 1.8 and 2.0 A (DD05.cpp:88-158). */
+double orc_pair_v(double rij, const double V[8]);
+void orc_bridge_coeffs(const double ends[6], double a[6]);
+/* Effective field sum_j J s_j + Landau + Heisenberg correction of every atom
+ * at the current spins (calculate_spin_CPU.cpp:111-140,201-274), no update. */
+void orc_fields(orc_ctx* o, double* h);
+
+/* ---- parametric potential model (sld_model_oracle.c) ----
+ * theta[22]: a_k[6] (density), V_k[8] (pair), aa, bb (embedding), J_A, J_rc
+ * (exchange), Landau A..D. Kinds of generator: */
+#define ORCM_EMBED 0   /* F(rho) */
+#define ORCM_DENS 1    /* f(r) */
+#define ORCM_PAIR 2    /* V(r) */
+#define ORCM_EXCH 3    /* J(r) */
+#define ORCM_NPARAM 22
+typedef struct orcm_tab orcm_tab;
+void orcm_default_params(double* theta);
+double orcm_gen(const double* theta, int kind, double x);
+/* d gen / d theta_k at x (zero when gen does not depend on theta_k) */
+double orcm_dgen(const double* theta, int kind, int k, double x);
+/* tables of the model at theta (10^6 samples, 6-point derivatives, as
+ * potential_CPU.cpp:61-170) and of the derivative of each generator */
+orcm_tab* orcm_build(const double* theta, double rcut_max);
+void orcm_free(orcm_tab* t);
+double orcm_table_raw(const orcm_tab* t, int kind, int deriv, int n);
+/* One configuration (orthorhombic box lengths L[3]): energy, forces and
+ * fields, and grad[k] += gE dE/dtheta_k + sum_i gF_i . dF_i/dtheta_k +
+ * gH_i . dH_i/dtheta_k, by forward-mode differentiation one parameter at a
+ * time. Returns 0, or -1 for a box shorter than twice rcut_max. */
+int orcm_eval_grad(const orcm_tab* t, const double L[3], int n, const double* r, const double* s,
+                   double rcut_max, double rcut_pot, double rcut_mag, double gE, const double* gF,
+                   const double* gH, double* E, double* F, double* H, double* grad);
+
 /* Philox4x32-10 (Salmon et al., SC'11) of one 128-bit counter. */
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Three N(0,1) draws of the counter-based noise stream for one atom and one

@@ -18,6 +18,7 @@
 
 #include "../../include/spilady_hip.h"
 #include "sld_kernels.h"
+#include "sld_model.h"
 #include "sld_tables.h"
 
 using namespace sld;
@@ -89,6 +90,18 @@ struct sld_ctx {
     double timed_ms = 0.0;
     std::string err;
     std::vector<void*> allocs;
+    // parametric potential model over the replicas' frozen configurations
+    struct Model {
+        bool ready = false;
+        double theta[sld::kNParam] = {};
+        sld::ModelTab tab{};
+        double *d_r = nullptr, *d_rd = nullptr, *d_h = nullptr;
+        double *d_rho = nullptr, *d_fp = nullptr, *d_Q = nullptr;
+        double *d_F = nullptr, *d_H = nullptr, *d_gE = nullptr, *d_gF = nullptr, *d_gH = nullptr;
+        double *d_E = nullptr, *d_grad = nullptr, *d_part = nullptr;
+        int* d_coff = nullptr;
+        int ntot = 0;
+    } model;
 };
 
 namespace {
@@ -538,6 +551,20 @@ int read_flags(sld_ctx* c) {
         return set_err(c, SLD_ERR_STATE, "an atom moved beyond the neighbouring link cells in one step");
     if (flags & kErrNonFinite)
         return set_err(c, SLD_ERR_STATE, "non-finite position, momentum or spin");
+    return SLD_OK;
+}
+
+// sums the event pairs of the timed kernel recorded since ev_used was reset
+// (the stream must have passed the last event)
+int collect_timing(sld_ctx* c) {
+    c->timed_launches = 0;
+    c->timed_ms = 0.0;
+    for (int k = 0; k + 1 < c->ev_used; k += 2) {
+        float kms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_pool[k], c->ev_pool[k + 1]));
+        c->timed_ms += kms;
+        c->timed_launches += c->ev_launches[k / 2];
+    }
     return SLD_OK;
 }
 
@@ -1025,14 +1052,7 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
     c->last_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->timed_launches = 0;
-    c->timed_ms = 0.0;
-    for (int k = 0; k + 1 < c->ev_used; k += 2) {
-        float kms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_pool[k], c->ev_pool[k + 1]));
-        c->timed_ms += kms;
-        c->timed_launches += c->ev_launches[k / 2];
-    }
+    TRY(collect_timing(c));
     if (obs && k > 0)
         HIPCHK(c, hipMemcpy(obs, c->d_obs, sizeof(sld_obs) * k * c->R, hipMemcpyDeviceToHost));
     if (nobs) *nobs = k;
@@ -1139,5 +1159,136 @@ int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
 }
 
 double sld_total_time(const sld_ctx* c) { return c->total_time; }
+
+// ---------------------------------------------------------------------------
+// parametric potential model (sld_model.h, sld_model.hip)
+
+int sld_model_nparams(void) { return kNParam; }
+
+int sld_model_default_params(double* theta) {
+    if (!theta) return SLD_ERR_ARG;
+    model_default_params(theta);
+    return SLD_OK;
+}
+
+int sld_model_set_params(sld_ctx* c, const double* theta) {
+    if (!c) return SLD_ERR_ARG;
+    if (!theta) return set_err(c, SLD_ERR_ARG, "null parameter vector");
+    for (int k = 0; k < kNParam; ++k)
+        if (!std::isfinite(theta[k])) return set_err(c, SLD_ERR_ARG, "parameter %d is not finite", k);
+    if (!(theta[P_JRC] > 0.0)) return set_err(c, SLD_ERR_ARG, "exchange cutoff must be positive");
+    auto& m = c->model;
+    const int N = c->D.N;
+    if (!m.d_r) {
+        TRY(dalloc(c, &m.d_r, (size_t)kNInput * kMRStride));
+        TRY(dalloc(c, &m.d_rd, (size_t)kNInput * kMRDStride));
+        TRY(dalloc(c, &m.d_h, (size_t)kNInput * kMHStride));
+        std::vector<int> coff(c->R);
+        int tot = 0;
+        for (int r = 0; r < c->R; ++r) {
+            coff[r] = tot;
+            tot += c->reps[r].natom;
+        }
+        m.ntot = tot;
+        TRY(upload_vec(c, &m.d_coff, coff));
+        TRY(dalloc(c, &m.d_rho, N)); TRY(dalloc(c, &m.d_fp, N)); TRY(dalloc(c, &m.d_Q, N));
+        TRY(dalloc(c, &m.d_F, (size_t)3 * tot)); TRY(dalloc(c, &m.d_H, (size_t)3 * tot));
+        TRY(dalloc(c, &m.d_gF, (size_t)3 * tot)); TRY(dalloc(c, &m.d_gH, (size_t)3 * tot));
+        TRY(dalloc(c, &m.d_gE, c->R)); TRY(dalloc(c, &m.d_E, (size_t)kScal * c->R));
+        TRY(dalloc(c, &m.d_grad, kNParam));
+        TRY(dalloc(c, &m.d_part, (size_t)nblk(N, 256) * kNParam));
+    }
+    ModelHostTables t;
+    build_model_tables(theta, c->prm.rcut_max, t);
+    HIPCHK(c, hipMemcpyAsync(m.d_r, t.r.data(), t.r.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(m.d_rd, t.rd.data(), t.rd.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(m.d_h, t.h.data(), t.h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    m.tab.r = m.d_r;
+    m.tab.rd = m.d_rd;
+    m.tab.h = m.d_h;
+    m.tab.in_over_rmax = t.in_over_rmax;
+    m.tab.in_over_rhomax = t.in_over_rhomax;
+    for (int k = 0; k < 4; ++k) m.tab.L[k] = theta[P_LA + k];
+    std::copy(theta, theta + kNParam, m.theta);
+    m.ready = true;
+    return SLD_OK;
+}
+
+int sld_model_get_params(sld_ctx* c, double* theta) {
+    if (!c || !theta) return SLD_ERR_ARG;
+    if (!c->model.ready) return set_err(c, SLD_ERR_STATE, "model parameters not set");
+    std::copy(c->model.theta, c->model.theta + kNParam, theta);
+    return SLD_OK;
+}
+
+namespace {
+// densities and F'(rho) of the current configurations at the current parameters
+int model_rho(sld_ctx* c) {
+    auto& m = c->model;
+    if (!m.ready) return set_err(c, SLD_ERR_STATE, "model parameters not set (sld_model_set_params)");
+    TRY(ensure_nb(c));
+    TRY(read_flags(c));
+    LAUNCH(c, k_model_rho, nblk(c->D.N, 256), 256, m.tab, c->phys, c->D, m.d_rho, m.d_fp);
+    return SLD_OK;
+}
+
+// copy between a caller's buffer (host or device) and a device buffer
+int copy_any(sld_ctx* c, void* dst, const void* src, size_t bytes) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+    return SLD_OK;
+}
+}  // namespace
+
+int sld_model_eval(sld_ctx* c, double* energy, double* force, double* field) {
+    if (!c) return SLD_ERR_ARG;
+    auto& m = c->model;
+    c->ev_used = 0;
+    TRY(model_rho(c));
+    const int nb = nblk(c->D.N, 256);
+    LAUNCH(c, k_model_forward, nb, 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
+           (const int*)m.d_coff, m.d_F, m.d_H, m.d_part);
+    LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)m.d_part, (const int*)c->d_atom_seg, 1, m.d_E, 0);
+    if (energy)
+        HIPCHK(c, hipMemcpy2DAsync(energy, sizeof(double), m.d_E, kScal * sizeof(double), sizeof(double), c->R,
+                                   hipMemcpyDefault, c->stream));
+    const size_t b3 = (size_t)3 * m.ntot * sizeof(double);
+    if (force) TRY(copy_any(c, force, m.d_F, b3));
+    if (field) TRY(copy_any(c, field, m.d_H, b3));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+}
+
+int sld_model_vjp(sld_ctx* c, const double* g_energy, const double* g_force, const double* g_field,
+                  double* grad) {
+    if (!c) return SLD_ERR_ARG;
+    if (!grad) return set_err(c, SLD_ERR_ARG, "null gradient output");
+    auto& m = c->model;
+    c->ev_used = 0;
+    TRY(model_rho(c));
+    const size_t b3 = (size_t)3 * m.ntot * sizeof(double);
+    if (g_energy) TRY(copy_any(c, m.d_gE, g_energy, c->R * sizeof(double)));
+    else HIPCHK(c, hipMemsetAsync(m.d_gE, 0, c->R * sizeof(double), c->stream));
+    if (g_force) TRY(copy_any(c, m.d_gF, g_force, b3));
+    else HIPCHK(c, hipMemsetAsync(m.d_gF, 0, b3, c->stream));
+    if (g_field) TRY(copy_any(c, m.d_gH, g_field, b3));
+    else HIPCHK(c, hipMemsetAsync(m.d_gH, 0, b3, c->stream));
+    const int nb = nblk(c->D.N, 256);
+    LAUNCH(c, k_model_q, nb, 256, m.tab, c->phys, c->D, (const int*)m.d_coff, (const double*)m.d_gF, m.d_Q);
+    LAUNCH(c, k_model_adjoint, nb, 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
+           (const double*)m.d_Q, (const int*)m.d_coff, (const double*)m.d_gE, (const double*)m.d_gF,
+           (const double*)m.d_gH, m.d_part);
+    LAUNCH(c, k_reduce, 1, 256, (const double*)m.d_part, nb, kNParam, m.d_grad);
+    TRY(copy_any(c, grad, m.d_grad, kNParam * sizeof(double)));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+}
+
+int sld_model_natoms(const sld_ctx* c) {
+    if (!c) return 0;
+    int tot = 0;
+    for (const Rep& r : c->reps) tot += r.natom;
+    return tot;
+}
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "sld_device.h"
+#include "sld_model.h"
 
 namespace sld {
 
@@ -82,6 +83,22 @@ __global__ void k_renorm(Dev D, const double* err, double* partials);
 __global__ void k_reduce(const double* partials, int nblk, int nk, double* out);
 __global__ void k_reduce_seg(const double* partials, const int* seg, int nk, double* out, int out_off);
 __global__ void k_check_nonfinite(Dev D);
+
+// ---- parametric potential model (sld_model.hip) ----
+struct ModelTab {
+    const double* r;    // [N][kMRStride]
+    const double* rd;   // [N][kMRDStride]
+    const double* h;    // [N][kMHStride]
+    double in_over_rmax, in_over_rhomax;
+    double L[4];        // Landau A B C D
+};
+__global__ void k_model_rho(ModelTab M, Phys P, Dev D, double* rho_out, double* fp_out);
+__global__ void k_model_forward(ModelTab M, Phys P, Dev D, const double* rho, const double* fp, const int* coff,
+                                double* F, double* H, double* partials);
+__global__ void k_model_q(ModelTab M, Phys P, Dev D, const int* coff, const double* gF, double* Q);
+__global__ void k_model_adjoint(ModelTab M, Phys P, Dev D, const double* rho, const double* fp, const double* Q,
+                                const int* coff, const double* gE, const double* gF, const double* gH,
+                                double* partials);
 
 constexpr int kObsAtomK = 11;  // pe ke me me0 sx sy sz rup rdn lup ldn
 constexpr int kObsCellK = 6;   // Ee sRup sRdn sLup sLdn Te

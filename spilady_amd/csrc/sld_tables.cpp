@@ -35,7 +35,9 @@ double smallf_gen(double r) {
     return f;
 }
 
-double zbl(double r) {
+}  // namespace
+
+double zbl_pair(double r) {
     const double b0 = 0.1818e0, b1 = 0.5099e0, b2 = 0.2802e0, b3 = 0.02817e0;
     const double z = 26e0;
     const double abohr = 0.52917720859e0;
@@ -46,28 +48,35 @@ double zbl(double r) {
     return z * z * 14.3992 * v / r;
 }
 
+namespace {
+
+double zbl(double r) { return zbl_pair(r); }
+
 const double kPV[8] = {-0.1960674387419232, 0.3687525935422963, -1.505333614924853,
                        4.948907078156191,   -4.894613262753399, 3.468897724782442,
                        -1.792218099820337,  80.22069592246987};
 const double kPR[8] = {4.1, 3.8, 3.5, 3.2, 2.9, 2.6, 2.4, 2.3};
 
-// Quintic bridge between ZBL at x1 and the knot spline at x2 matching value,
-// slope and curvature at both ends (DD05.cpp:93-150). The coefficients are
-// the closed-form solution of that 6x6 Hermite system.
-struct Bridge {
-    double a[6];
-};
+}  // namespace
 
-Bridge make_bridge() {
+// Quintic bridge between ZBL at x1 = 1.8 and the knot spline at x2 = 2.0,
+// matching value, slope and curvature at both ends (DD05.cpp:93-150): the
+// closed-form solution of that 6x6 Hermite system. It is linear in the six
+// end values.
+ZblEnds zbl_ends() {
+    using std::pow;
+    const double x1 = 1.8;
+    ZblEnds e;
+    e.y1 = zbl_pair(x1);
+    e.Dy1 = (zbl_pair(x1 + 1e-7) - zbl_pair(x1 - 1e-7)) / 2e-7;
+    e.D2y1 = (zbl_pair(x1 + 1e-7) + zbl_pair(x1 - 1e-7) - 2e0 * zbl_pair(x1)) / pow(1e-7, 2);
+    return e;
+}
+
+Bridge bridge_from_ends(const ZblEnds& e, double y2, double Dy2, double D2y2) {
     using std::pow;
     const double x1 = 1.8, x2 = 2.0;
-    double y1 = zbl(x1);
-    double Dy1 = (zbl(x1 + 1e-7) - zbl(x1 - 1e-7)) / 2e-7;
-    double D2y1 = (zbl(x1 + 1e-7) + zbl(x1 - 1e-7) - 2e0 * zbl(x1)) / pow(1e-7, 2);
-    double y2 = 0e0, Dy2 = 0e0, D2y2 = 0e0;
-    for (int i = 0; i < 8; ++i) if (x2 < kPR[i]) y2 += kPV[i] * pow((kPR[i] - x2), 3);
-    for (int i = 0; i < 8; ++i) if (x2 < kPR[i]) Dy2 -= 3e0 * kPV[i] * pow((kPR[i] - x2), 2);
-    for (int i = 0; i < 8; ++i) if (x2 < kPR[i]) D2y2 += 6e0 * kPV[i] * (kPR[i] - x2);
+    const double y1 = e.y1, Dy1 = e.Dy1, D2y1 = e.D2y1;
     const double den = 2e0 * pow((x1 - x2), 5);
     Bridge b;
     b.a[5] = (-6e0*Dy1*x1-6e0*Dy2*x1+D2y1*pow(x1,2)-D2y2*pow(x1,2)
@@ -106,10 +115,24 @@ Bridge make_bridge() {
     return b;
 }
 
+
+namespace {
+
+Bridge make_bridge() {
+    using std::pow;
+    const double x2 = 2.0;
+    double y2 = 0e0, Dy2 = 0e0, D2y2 = 0e0;
+    for (int i = 0; i < 8; ++i) if (x2 < kPR[i]) y2 += kPV[i] * pow((kPR[i] - x2), 3);
+    for (int i = 0; i < 8; ++i) if (x2 < kPR[i]) Dy2 -= 3e0 * kPV[i] * pow((kPR[i] - x2), 2);
+    for (int i = 0; i < 8; ++i) if (x2 < kPR[i]) D2y2 += 6e0 * kPV[i] * (kPR[i] - x2);
+    return bridge_from_ends(zbl_ends(), y2, Dy2, D2y2);
+}
+
 const Bridge& bridge() {
     static const Bridge b = make_bridge();
     return b;
 }
+
 
 double pair_gen(double r) {
     using std::pow;

@@ -13,6 +13,17 @@ struct HostTables {
 };
 
 void build_tables(double rcut_max, HostTables& t);
+
+// pieces of the DD05 pair function shared with the parametric model
+double zbl_pair(double r);                 // DD05.cpp:45-62
+struct ZblEnds {
+    double y1, Dy1, D2y1;                  // ZBL value, slope, curvature at r = 1.8
+};
+struct Bridge {
+    double a[6];                           // quintic coefficients, a[k] r^k
+};
+ZblEnds zbl_ends();
+Bridge bridge_from_ends(const ZblEnds& e, double y2, double Dy2, double D2y2);
 // The tables for rcut_max, built once per process and shared by every
 // context created with the same cutoff.
 std::shared_ptr<const HostTables> shared_tables(double rcut_max);

@@ -39,12 +39,26 @@ def exported_symbols() -> List[str]:
     return sorted(set(re.findall(r"\b(sld_[a-z_0-9]+)\s*\(", txt)))
 
 
+def _one_hip_runtime():
+    """torch's ROCm wheel carries its own libamdhip64, which it loads under a
+    file name that does not match the system runtime's soname. A process that
+    loads this library first and torch afterwards holds two HIP runtimes, and
+    the second finds no GPU. Importing torch first (when it is installed)
+    makes this library bind to the runtime torch loaded, so device tensors
+    and the library share one runtime in every import order."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path: Optional[str] = None):
     """Load the HIP library (no device call)."""
     global _lib
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    _one_hip_runtime()
     if not os.path.exists(p):
         raise FileNotFoundError(f"{p} is not built: run __graft_entry__.build() "
                                 "(make -C spilady_amd/csrc)")
@@ -80,6 +94,14 @@ def load(path: Optional[str] = None):
     L.sld_last_step_timing.argtypes = [V, _dp]
     L.sld_time_kernel.argtypes = [V, C.c_char_p]
     L.sld_kernel_timing.argtypes = [V, _ip, _dp]
+    L.sld_model_nparams.argtypes = []
+    L.sld_model_default_params.argtypes = [_dp]
+    L.sld_model_set_params.argtypes = [V, _dp]
+    L.sld_model_get_params.argtypes = [V, _dp]
+    L.sld_model_natoms.argtypes = [V]
+    # buffers are raw addresses: host arrays or device tensors
+    L.sld_model_eval.argtypes = [V, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sld_model_vjp.argtypes = [V, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     if path is None:
         _lib = L
     return L
