@@ -212,17 +212,24 @@ def batched_vs_serial(steps: int, warmup: int):
     natoms = sum(p.natom for p, _ in reps)
     b = BatchEngine([p for p, _ in reps], [a for _, a in reps], compute_initial_forces=True)
     b.step(0, warmup)
+    # untuned batched rate, then the autotuner (state restored), then the
+    # timed tuned run that the serial one is compared with
     t0 = time.perf_counter()
-    ob = b.step(warmup, steps)
+    b.step(warmup, steps)
+    t_default = time.perf_counter() - t0
+    default_shapes = b.shapes()
+    tuned, table = b.autotune(2)
+    t0 = time.perf_counter()
+    ob = b.step(warmup + steps, steps)
     tb = time.perf_counter() - t0
     b.close()
     ts = 0.0
     equal = True
     for r, (p, a) in enumerate(reps):
         e = Engine(p, a, compute_initial_forces=True)
-        e.step(0, warmup)
+        e.step(0, warmup + steps)
         t0 = time.perf_counter()
-        os_ = e.step(warmup, steps)
+        os_ = e.step(warmup + steps, steps)
         ts += time.perf_counter() - t0
         e.close()
         equal = equal and all(np.array_equal(ob[k][r][x], os_[k][x]) for k in range(steps) for x in os_[k])
@@ -230,6 +237,8 @@ def batched_vs_serial(steps: int, warmup: int):
             "batched_atom_steps_per_s": natoms * steps / tb,
             "serial_atom_steps_per_s": natoms * steps / ts,
             "speedup": ts / tb, "observables_bitwise_equal": equal,
+            "untuned_batched_atom_steps_per_s": natoms * steps / t_default,
+            "autotune": {"default": default_shapes, "chosen": tuned, "us_per_step": table},
             "workload": "64 bcc Fe replicas, L^3 cubic cells, L uniform in {8..32}, 100-1400 K, "
                         "seed 100+r, Langevin lattice+spin, dt 1 fs"}
 
@@ -284,6 +293,8 @@ def main() -> int:
                     help="instead of the benchmark, measure NVE energy drift over this many steps")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="timed steps of the batched-vs-serial replica comparison (0: skip)")
+    ap.add_argument("--autotune-steps", type=int, default=2,
+                    help="steps per candidate launch shape in the autotuner before timing (0: default shapes)")
     ap.add_argument("--adjoint-reps", type=int, default=10,
                     help="timed loss gradients of the potential-fitting workload (0: skip)")
     args = ap.parse_args()
@@ -301,6 +312,11 @@ def main() -> int:
     eng = Engine(p, at, compute_initial_forces=True)
     dev, ncu = device_info()
     eng.step(0, args.warmup)
+    tune = None
+    if args.autotune_steps > 0:
+        default_shapes = eng.shapes()
+        chosen, table = eng.autotune(args.autotune_steps)
+        tune = {"default": default_shapes, "chosen": chosen, "us_per_step": table}
     t0 = time.perf_counter()
     eng.step(args.warmup, args.steps)
     t1 = time.perf_counter()
@@ -381,6 +397,7 @@ def main() -> int:
         "roofline": roof,
         "kernels": kern,
         "step_roofline": step_roof,
+        "autotune": tune,
         "batched": batched,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

@@ -187,6 +187,30 @@ int sld_last_step_timing(sld_ctx* ctx, double* total_ms);
 int sld_time_kernel(sld_ctx* ctx, const char* kernel);
 int sld_kernel_timing(sld_ctx* ctx, int* launches, double* total_ms);
 
+/* ---- Launch shapes and the autotuner ----
+ * No counterpart in the reference (its CUDA launches use fixed block sizes).
+ * These are the block shapes of the kernels whose results do not depend on
+ * them: every reduction inside them sums per wavefront, so a trajectory is
+ * bitwise the same under any valid shapes. */
+typedef struct {
+    int spin_block;        /* core_ds sweep level: threads (one per cell) per block; 32, 64 or 128 */
+    int sweep_block;       /* core_dTe diffusion level: threads per block; 64, 128 or 256 */
+    int force_block;       /* force/energy: threads (one per atom) per block; 64, 128 or 256 */
+    int nbuild_cells;      /* neighbour build: link cells (one wave each) per block; 1, 2 or 4 */
+    int temperature_block; /* calculate_temperature: threads (32 per cell) per block; 64, 128 or 256 */
+} sld_shapes;
+int sld_get_shapes(sld_ctx* ctx, sld_shapes* out);
+int sld_set_shapes(sld_ctx* ctx, const sld_shapes* in);
+/* Times each candidate shape of each kernel over `steps` full steps (after
+ * one warm-up step) with HIP events, keeps the fastest, and restores the
+ * context's state afterwards, so the trajectory continues exactly as if the
+ * call had not been made. best (may be NULL) receives the chosen shapes;
+ * us (may be NULL, 15 entries: kernel-major in the order of sld_shapes, three
+ * candidates each in ascending order) the device microseconds per step of
+ * each candidate, 0 where the kernel did not run. Needs device memory for a
+ * copy of the context's state while it runs. */
+int sld_autotune(sld_ctx* ctx, int steps, sld_shapes* best, double* us);
+
 /* ---- Parametric potential model and its parameter adjoint ----
  * No counterpart in the reference, whose potential (DD05.cpp, JijFe.cpp) is
  * fixed at compile time; these are for callers that fit the potential's

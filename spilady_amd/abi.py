@@ -60,6 +60,16 @@ class SldCells(C.Structure):
                                    "sum_r_dn", "sum_l_up", "sum_l_dn")]
 
 
+class SldShapes(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("spin_block", "sweep_block", "force_block", "nbuild_cells",
+                                         "temperature_block")]
+
+
+SHAPE_CANDIDATES = {"spin_block": (32, 64, 128), "sweep_block": (64, 128, 256),
+                    "force_block": (64, 128, 256), "nbuild_cells": (1, 2, 4),
+                    "temperature_block": (64, 128, 256)}
+
+
 VEC_FIELDS = ("r", "p", "f", "s", "heff_h", "heff_l")
 SCALAR_FIELDS = ("rho", "pe", "ke", "me", "me0", "vir", "sum_jij_sj")
 CELL_FIELDS = ("te", "ee", "tl", "ts_r", "ts_l", "gel", "ges", "sum_r_up", "sum_r_dn",

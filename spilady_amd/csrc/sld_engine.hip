@@ -90,6 +90,10 @@ struct sld_ctx {
     double timed_ms = 0.0;
     std::string err;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // size of each entry of allocs
+    // launch shapes of the kernels whose results do not depend on them
+    // (defaults: the autotuner's choice on the 2M-atom step on MI355X)
+    sld_shapes shapes{64, 64, 256, 2, 256};
     // parametric potential model over the replicas' frozen configurations
     struct Model {
         bool ready = false;
@@ -129,6 +133,7 @@ int dalloc(sld_ctx* c, T** p, size_t n) {
     HIPCHK(c, hipMalloc(&q, n * sizeof(T) + 64));
     HIPCHK(c, hipMemsetAsync(q, 0, n * sizeof(T) + 64, c->stream));
     c->allocs.push_back(q);
+    c->alloc_bytes.push_back(n * sizeof(T) + 64);
     *p = (T*)q;
     return SLD_OK;
 }
@@ -295,7 +300,8 @@ int reduce_seg(sld_ctx* c, const int* d_seg, int nk, int out_off) {
 }
 
 int nbuild(sld_ctx* c, int write_rho) {
-    LAUNCH(c, k_nbuild, nblk(c->D.C, 4), 256, c->tab, c->phys, c->D, write_rho);
+    const int w = c->shapes.nbuild_cells;
+    LAUNCH(c, k_nbuild, nblk(c->D.C, w), 64 * w, c->tab, c->phys, c->D, write_rho);
     c->nb_dirty = false;
     return SLD_OK;
 }
@@ -317,7 +323,7 @@ int do_links(sld_ctx* c) {
 int do_force(sld_ctx* c) {
     TRY(ensure_nb(c));
     LAUNCH(c, k_fp, nblk(c->D.N, 256), 256, c->tab, c->D);
-    LAUNCH(c, k_force, nblk(c->D.N, 256), 256, c->tab, c->phys, c->D);
+    LAUNCH(c, k_force, nblk(c->D.N, c->shapes.force_block), c->shapes.force_block, c->tab, c->phys, c->D);
     return SLD_OK;
 }
 
@@ -326,9 +332,11 @@ int do_ds(sld_ctx* c, double dt) {
     double dth = dt / 2.0;
     for (int dir = 0; dir < 2; ++dir) {
         LevelSet& L = dir == 0 ? c->spin_fwd : c->spin_bwd;
+        if (c->phys.spinlang) LAUNCH(c, k_spin_noise, nblk(c->D.N, 256), 256, c->D, c->pass);
         for (int l = 0; l < L.nlev(); ++l) {
             int n = L.off[l + 1] - L.off[l];
-            LAUNCH(c, k_spin_phase, nblk(n, 128), 128, L.d_cells + L.off[l], n, dir, dth, c->tab,
+            const int bs = c->shapes.spin_block;
+            LAUNCH(c, k_spin_phase, nblk(n, bs), bs, L.d_cells + L.off[l], n, dir, dth, c->tab,
                    c->phys, c->D, c->pass);
         }
         c->pass++;
@@ -358,7 +366,8 @@ int do_dr(sld_ctx* c, double dt) {
 
 int do_temperature(sld_ctx* c) {
     TRY(ensure_nb(c));
-    LAUNCH(c, k_temperature, nblk(c->D.C, 8), 256, c->tab, c->phys, c->D);
+    const int bs = c->shapes.temperature_block;
+    LAUNCH(c, k_temperature, nblk(c->D.C, bs / 32), bs, c->tab, c->phys, c->D);
     return SLD_OK;
 }
 
@@ -421,9 +430,10 @@ int do_dTe_launches(sld_ctx* c, double dt) {
             for (int l = 0; l < L.nlev(); ++l) {
                 int n = L.off[l + 1] - L.off[l];
                 double* part = dir == 1 ? c->d_part + (size_t)nb + npb : nullptr;
-                LAUNCH(c, k_dte_sweep, nblk(n, 256), 256, L.d_cells + L.off[l], n, pref, h0.dx2,
+                const int bs = c->shapes.sweep_block;
+                LAUNCH(c, k_dte_sweep, nblk(n, bs), bs, L.d_cells + L.off[l], n, pref, h0.dx2,
                        h0.dy2, h0.dz2, c->reps[0].box, c->D, part);
-                if (dir == 1) npb += nblk(n, 256);
+                if (dir == 1) npb += nblk(n, 64);  // one partial per wave
                 nlaunch += n > 0;
             }
         }
@@ -785,6 +795,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     TRY(dalloc(c, &D.s, N)); TRY(dalloc(c, &D.hh, N)); TRY(dalloc(c, &D.hl, N));
     TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
     TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
+    TRY(dalloc(c, &D.zn, N));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
     TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
@@ -1027,6 +1038,7 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
         sld_obs* nb = nullptr;
         HIPCHK(c, hipMalloc(&nb, sizeof(sld_obs) * nprint * c->R));
         c->allocs.push_back(nb);
+        c->alloc_bytes.push_back(sizeof(sld_obs) * nprint * c->R);
         c->d_obs = nb;
         c->obs_cap = nprint;
     }
@@ -1161,6 +1173,167 @@ int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
 double sld_total_time(const sld_ctx* c) { return c->total_time; }
 
 // ---------------------------------------------------------------------------
+// launch shapes and the autotuner
+
+}  // extern "C"
+
+namespace {
+const int kShapeCand[5][3] = {{32, 64, 128}, {64, 128, 256}, {64, 128, 256}, {1, 2, 4}, {64, 128, 256}};
+const char* kShapeKernel[5] = {"k_spin_phase", "k_dte_sweep", "k_force", "k_nbuild", "k_temperature"};
+
+int& shape_field(sld_shapes& s, int k) {
+    switch (k) {
+        case 0: return s.spin_block;
+        case 1: return s.sweep_block;
+        case 2: return s.force_block;
+        case 3: return s.nbuild_cells;
+        default: return s.temperature_block;
+    }
+}
+
+bool shape_valid(const sld_shapes& s) {
+    sld_shapes t = s;
+    for (int k = 0; k < 5; ++k) {
+        const int v = shape_field(t, k);
+        if (v != kShapeCand[k][0] && v != kShapeCand[k][1] && v != kShapeCand[k][2]) return false;
+    }
+    return true;
+}
+
+void drop_dte_graph(sld_ctx* c) {
+    if (c->dte_exec) {
+        (void)hipGraphExecDestroy(c->dte_exec);
+        c->dte_exec = nullptr;
+    }
+}
+
+// a device copy of every allocation and the host fields that a step changes
+struct Snapshot {
+    void* buf = nullptr;
+    Dev D{};
+    int *cl2 = nullptr, *cn2 = nullptr;
+    uint64_t pass = 0;
+    double total_time = 0.0;
+    bool nb_dirty = true;
+};
+
+int snapshot_take(sld_ctx* c, Snapshot& s) {
+    size_t tot = 0;
+    for (size_t b : c->alloc_bytes) tot += (b + 255) & ~size_t(255);
+    HIPCHK(c, hipMalloc(&s.buf, tot));
+    size_t off = 0;
+    for (size_t k = 0; k < c->allocs.size(); ++k) {
+        HIPCHK(c, hipMemcpyAsync((char*)s.buf + off, c->allocs[k], c->alloc_bytes[k], hipMemcpyDeviceToDevice,
+                                 c->stream));
+        off += (c->alloc_bytes[k] + 255) & ~size_t(255);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s.D = c->D;
+    s.cl2 = c->d_cl2;
+    s.cn2 = c->d_cn2;
+    s.pass = c->pass;
+    s.total_time = c->total_time;
+    s.nb_dirty = c->nb_dirty;
+    return SLD_OK;
+}
+
+int snapshot_restore(sld_ctx* c, Snapshot& s) {
+    if (!s.buf) return SLD_OK;
+    (void)hipStreamSynchronize(c->stream);
+    size_t off = 0;
+    for (size_t k = 0; k < c->allocs.size(); ++k) {
+        HIPCHK(c, hipMemcpyAsync(c->allocs[k], (char*)s.buf + off, c->alloc_bytes[k], hipMemcpyDeviceToDevice,
+                                 c->stream));
+        off += (c->alloc_bytes[k] + 255) & ~size_t(255);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(s.buf));
+    s.buf = nullptr;
+    c->D = s.D;
+    c->d_cl2 = s.cl2;
+    c->d_cn2 = s.cn2;
+    c->pass = s.pass;
+    c->total_time = s.total_time;
+    c->nb_dirty = s.nb_dirty;
+    return SLD_OK;
+}
+
+// device microseconds per step spent in kernel `name` over `steps` steps
+int time_in_steps(sld_ctx* c, const char* name, int steps, double* us, int* launches) {
+    c->timed_name = name;
+    c->ev_used = 0;
+    for (int k = 0; k < steps; ++k) TRY(do_core(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    TRY(collect_timing(c));
+    c->timed_name.clear();
+    *us = c->timed_ms * 1e3 / steps;
+    *launches = c->timed_launches;
+    return SLD_OK;
+}
+
+int autotune_body(sld_ctx* c, int steps, double* us) {
+    TRY(do_core(c));  // warm-up
+    for (int k = 0; k < 5; ++k) {
+        double best = 0.0;
+        int best_v = shape_field(c->shapes, k);
+        for (int q = 0; q < 3; ++q) {
+            shape_field(c->shapes, k) = kShapeCand[k][q];
+            double t = 0.0;
+            int n = 0;
+            TRY(time_in_steps(c, kShapeKernel[k], steps, &t, &n));
+            if (us) us[3 * k + q] = n > 0 ? t : 0.0;
+            if (n > 0 && (q == 0 || t < best)) {
+                best = t;
+                best_v = kShapeCand[k][q];
+            }
+        }
+        shape_field(c->shapes, k) = best_v;
+    }
+    return read_flags(c);
+}
+}  // namespace
+
+extern "C" {
+
+int sld_get_shapes(sld_ctx* c, sld_shapes* out) {
+    if (!c || !out) return SLD_ERR_ARG;
+    *out = c->shapes;
+    return SLD_OK;
+}
+
+int sld_set_shapes(sld_ctx* c, const sld_shapes* in) {
+    if (!c || !in) return SLD_ERR_ARG;
+    if (!shape_valid(*in)) return set_err(c, SLD_ERR_ARG, "launch shape outside the supported candidates");
+    c->shapes = *in;
+    drop_dte_graph(c);
+    return SLD_OK;
+}
+
+int sld_autotune(sld_ctx* c, int steps, sld_shapes* best, double* us) {
+    if (!c) return SLD_ERR_ARG;
+    if (steps < 1) return set_err(c, SLD_ERR_ARG, "autotune needs at least one timed step");
+    if (us)
+        for (int k = 0; k < 15; ++k) us[k] = 0.0;
+    const sld_shapes before = c->shapes;
+    Snapshot snap;
+    TRY(snapshot_take(c, snap));
+    drop_dte_graph(c);
+    int rc = autotune_body(c, steps, us);
+    c->timed_name.clear();
+    const std::string msg = c->err;
+    int rc2 = snapshot_restore(c, snap);
+    drop_dte_graph(c);
+    if (rc != SLD_OK) {
+        c->shapes = before;
+        c->err = msg;
+        return rc;
+    }
+    if (rc2 != SLD_OK) return rc2;
+    if (best) *best = c->shapes;
+    return SLD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // parametric potential model (sld_model.h, sld_model.hip)
 
 int sld_model_nparams(void) { return kNParam; }
@@ -1222,6 +1395,8 @@ int sld_model_get_params(sld_ctx* c, double* theta) {
     return SLD_OK;
 }
 
+}  // extern "C"
+
 namespace {
 // densities and F'(rho) of the current configurations at the current parameters
 int model_rho(sld_ctx* c) {
@@ -1239,6 +1414,8 @@ int copy_any(sld_ctx* c, void* dst, const void* src, size_t bytes) {
     return SLD_OK;
 }
 }  // namespace
+
+extern "C" {
 
 int sld_model_eval(sld_ctx* c, double* energy, double* force, double* field) {
     if (!c) return SLD_ERR_ARG;

@@ -33,6 +33,7 @@ struct Dev {
     const int* arep;           // replica of each atom slot, -1 for padding [N]
     const int* crep;           // replica of each cell, -1 for padding [C]
     double4 *r, *p, *f, *s, *hh, *hl;
+    double4* zn;               // Langevin normals of the current spin pass [N]
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
     double* stress;            // [6][N]: 11 22 33 12 23 31
     int* cell;                 // link cell of each atom (new_cell_index)
@@ -64,6 +65,7 @@ __global__ void k_relink(const int* cl_old, const int* cn_old, Dev D);
 __global__ void k_nbuild(Tables T, Phys P, Dev D, int write_rho);
 __global__ void k_fp(Tables T, Dev D);
 __global__ void k_force(Tables T, Phys P, Dev D);
+__global__ void k_spin_noise(Dev D, uint64_t pass);
 __global__ void k_spin_phase(const int* cells, int ncell, int backward, double dt, Tables T, Phys P,
                              Dev D, uint64_t pass);
 __global__ void k_dp_cells(double dt, Phys P, Dev D, uint64_t pass, double* partials);

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(128) void k_relink(const int* cl_old, const int* cn
 // compacted with a ballot and a lane prefix count, so each list keeps the
 // reference's scan order.
 
-constexpr int kNbWaves = 4;                 // waves (cells) per block
+constexpr int kNbWaves = 4;                 // most waves (cells) per block
 constexpr int kMaxCand = 27 * kCellCap;     // upper bound of candidates
 constexpr int kCandReg = 4;                 // passes of 64 candidates held in registers
 
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
     __shared__ int s_cand[kNbWaves][kMaxCand];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int c = blockIdx.x * kNbWaves + w;
+    const int c = blockIdx.x * (blockDim.x >> 6) + w;
     const int rr = __builtin_amdgcn_readfirstlane(c < D.C ? D.crep[c] : -1);
     const bool valid = rr >= 0;
     Box b{};
@@ -455,11 +455,16 @@ __device__ __forceinline__ void exchange_field(int a, const Dev& D, double3& H, 
 // head to tail (forward) or tail to head (backward), one thread per cell
 // (calculate_spin_CPU.cpp:33-235, core_ds_CPU.cpp:35-83).
 //
-// The per-atom work is software pipelined so that the neighbour gathers run
-// under the arithmetic of the previous atom: during the update of atom q the
-// thread already holds atom q+1's neighbour ids and J values and has issued
-// the loads of those neighbours' spins; atom q+2's ids are in flight. The
-// only spin such a load can miss is atom q's own new value (written after
+// A level holds a few thousand cells, so each thread's chain of dependent
+// work is the level's duration. Everything that does not depend on other
+// atoms' new spins is taken off that chain:
+// - the Langevin normals of the pass come from k_spin_noise, a fully
+//   parallel kernel launched before the sweep;
+// - the per-atom work is software pipelined: during the update of atom q the
+//   thread already holds atom q+1's neighbour ids and J values, has issued
+//   the loads of those neighbours' spins and of atom q+1's own spin, noise
+//   and Landau coefficients, and has atom q+2's ids and density in flight.
+// The only spin such a load can miss is atom q's own new value (written after
 // the load was issued), which is patched in from registers, so every atom
 // sees exactly the spins the sequential sweep shows it.
 
@@ -513,24 +518,57 @@ __device__ __forceinline__ double3 apply_rot(const Rot& r, double3 s) {
     return t;
 }
 
-__global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell, int backward, double dt,
+// The Langevin normals of one spin pass for every atom (the counter-based
+// stream of normal3, keyed by replica seed and stream id, counted by the
+// atom's index in its replica and the pass).
+__global__ __launch_bounds__(256) void k_spin_noise(Dev D, uint64_t pass) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.N) return;
+    const int rr = D.arep[i];
+    if (rr < 0) return;
+    const Rep& R = D.rep[rr];
+    const double3 z = normal3(R.seed, R.replica, i - R.atom0, pass);
+    D.zn[i] = make_double4(z.x, z.y, z.z, 0.0);
+}
+
+// Landau coefficients A..D of an atom: interpolation weights of its density,
+// then the two table rows (calculate_spin_CPU.cpp:118-121)
+struct LanRows {
+    double4 lo, hi;
+    double ratio;
+};
+
+__device__ __forceinline__ LanRows landau_rows(const Tables& T, double rho) {
+    int n0;
+    double ratio;
+    interp_index(rho, T.in_over_rhomax, n0, ratio);
+    // columns A..D sit 16-byte aligned in 128-byte rows
+    const double2* lo = reinterpret_cast<const double2*>(T.rhotab + (size_t)(n0 - 1) * kRhoStride + H_A);
+    const double2* hi = reinterpret_cast<const double2*>(T.rhotab + (size_t)n0 * kRhoStride + H_A);
+    const double2 l0 = lo[0], l1 = lo[1], h0 = hi[0], h1 = hi[1];
+    LanRows L;
+    L.lo = make_double4(l0.x, l0.y, l1.x, l1.y);
+    L.hi = make_double4(h0.x, h0.y, h1.x, h1.y);
+    L.ratio = ratio;
+    return L;
+}
+
+__global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell, int backward, double dt,
                                                     Tables T, Phys P, Dev D, uint64_t pass) {
+    (void)pass;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncell) return;
     const int c = cells[g];
     const int n = D.cn[c];
     if (n == 0) return;
     const int* lst = D.cl + c * kCellCap;
-    const Rep& R = D.rep[D.crep[c]];
-    const int atom0 = R.atom0;
-    const uint64_t seed = R.seed;
-    const int stream = R.replica;
-    const double Tk = P.spinlang ? (P.eltemp ? D.Te[c] : R.temperature) : 0.0;
+    const bool lang = P.spinlang != 0;
+    const double Tk = lang ? (P.eltemp ? D.Te[c] : D.rep[D.crep[c]].temperature) : 0.0;
     const double dth = dt / 2.0, dtq = dt / 4.0, gS = P.gamma_S;
-    const double rs = P.spinlang ? sqrt(2.0 * Tk * gS / dt) : 0.0;
+    const double rs = lang ? sqrt(2.0 * Tk * gS / dt) : 0.0;
     auto atom_at = [&](int q) { return lst[backward ? n - 1 - q : q]; };
 
-    // atom q: ids/J/spins in registers; atom q+1: ids in registers
+    // pipeline registers: atom q (_c) and atom q+1 (_n)
     int a_c = atom_at(0), nm_c = D.nbmn[a_c];
     int id_c[kPipe], id_n[kPipe];
     double J_c[kPipe];
@@ -539,11 +577,16 @@ __global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell,
     load_js(D, a_c, J_c);
 #pragma unroll
     for (int k = 0; k < kPipe; ++k) sj[k] = D.s[k < nm_c ? id_c[k] : a_c];
+    double4 own_c = D.s[a_c];
+    double4 z_c = lang ? D.zn[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
+    LanRows lan_c = lang ? landau_rows(T, D.rho[a_c]) : LanRows{};
     int a_n = a_c, nm_n = nm_c;
+    double rho_n = 0.0;
     if (n > 1) {
         a_n = atom_at(1);
         nm_n = D.nbmn[a_n];
         load_ids(D, a_n, id_n);
+        rho_n = lang ? D.rho[a_n] : 0.0;
     }
     int prev_a = -1;
     double4 prev_s = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -566,10 +609,11 @@ __global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell,
             S += Jk * v.w;
         }
         const double3 H = make_double3(hx, hy, hz);
-        // the inputs of this atom that do not depend on other atoms
-        double3 s = d3(D.s[a]);
-        const double rho = D.rho[a];
-        // pipeline: issue atom q+1's spin and J loads and atom q+2's ids
+        double3 s = d3(own_c);
+        const double3 z = d3(z_c);
+        const LanRows lan = lan_c;
+        // pipeline: issue atom q+1's neighbour spins, J values, own spin,
+        // noise and Landau rows, and atom q+2's ids and density
         if (q + 1 < n) {
 #pragma unroll
             for (int k = 0; k < kPipe; ++k) {
@@ -577,26 +621,29 @@ __global__ __launch_bounds__(128) void k_spin_phase(const int* cells, int ncell,
                 sj[k] = D.s[k < nm_n ? id_n[k] : a_n];
             }
             load_js(D, a_n, J_c);
+            own_c = D.s[a_n];
+            if (lang) {
+                z_c = D.zn[a_n];
+                lan_c = landau_rows(T, rho_n);
+            }
             a_c = a_n;
             nm_c = nm_n;
             if (q + 2 < n) {
                 a_n = atom_at(q + 2);
                 nm_n = D.nbmn[a_n];
                 load_ids(D, a_n, id_n);
+                if (lang) rho_n = D.rho[a_n];
             }
         }
         double3 L = make_double3(0.0, 0.0, 0.0);
-        if (!P.spinlang) {
+        if (!lang) {
             s = apply_rot(make_rot(H, dt), s);
         } else {
-            double3 z = normal3(seed, stream, a - atom0, pass);
-            int n0;
-            double ratio;
-            interp_index(rho, T.in_over_rhomax, n0, ratio);
-            double A = lerp_tab(T.rhotab, kRhoStride, H_A, n0, ratio);
-            double B = lerp_tab(T.rhotab, kRhoStride, H_B, n0, ratio);
-            double C = lerp_tab(T.rhotab, kRhoStride, H_C, n0, ratio);
-            double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
+            const double w = lan.ratio, v = 1.0 - lan.ratio;
+            const double A = lan.lo.x * w + lan.hi.x * v;
+            const double B = lan.lo.y * w + lan.hi.y * v;
+            const double C = lan.lo.z * w + lan.hi.z * v;
+            const double Dd = lan.lo.w * w + lan.hi.w * v;
             const Rot R = make_rot(H, dth);
             s = apply_rot(R, s);
             L = landau_field(s, A, B, C, Dd, S);
@@ -860,7 +907,8 @@ __global__ __launch_bounds__(256) void k_dte_ab(double h, Phys P, Dev D, double*
 // cell's last update is that value up to one rounding. The six face
 // neighbours (map_cells.cpp slots 0/13, 2/14, 12/15) come from the cell index.
 // In the backward sweep every cell is updated exactly once, last, so the
-// launch also leaves block partials of the final energies (partials != null).
+// launch also leaves partials of the final energies (partials != null), one
+// per wave of 64 level entries: the sums do not depend on the block size.
 __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref, double dx2,
                                                    double dy2, double dz2, Box b, Dev D, double* partials) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -884,7 +932,10 @@ __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, 
         D.Te[c] = ee_to_te(ee);
         acc[0] = ee;
     }
-    if (partials) block_partials<1>(acc, partials);
+    if (partials) {
+        const double v = wave_sum(acc[0]);
+        if ((threadIdx.x & 63) == 0) partials[t >> 6] = v;
+    }
 }
 
 // the two totals of a sub-step: out[0] = sum of the starting-energy partials,

@@ -94,6 +94,10 @@ def load(path: Optional[str] = None):
     L.sld_last_step_timing.argtypes = [V, _dp]
     L.sld_time_kernel.argtypes = [V, C.c_char_p]
     L.sld_kernel_timing.argtypes = [V, _ip, _dp]
+    Sh = C.POINTER(abi.SldShapes)
+    L.sld_get_shapes.argtypes = [V, Sh]
+    L.sld_set_shapes.argtypes = [V, Sh]
+    L.sld_autotune.argtypes = [V, C.c_int, Sh, _dp]
     L.sld_model_nparams.argtypes = []
     L.sld_model_default_params.argtypes = [_dp]
     L.sld_model_set_params.argtypes = [V, _dp]
@@ -205,6 +209,29 @@ class Engine(abi.EngineLike):
         ms = C.c_double(0.0)
         self._rc(self.L.sld_kernel_timing(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def shapes(self) -> Dict[str, int]:
+        """Launch shapes of the shape-independent kernels."""
+        sh = abi.SldShapes()
+        self._rc(self.L.sld_get_shapes(self.h, C.byref(sh)))
+        return {k: int(getattr(sh, k)) for k, _ in abi.SldShapes._fields_}
+
+    def set_shapes(self, shapes: Dict[str, int]):
+        sh = abi.SldShapes(**{k: int(v) for k, v in {**self.shapes(), **shapes}.items()})
+        self._rc(self.L.sld_set_shapes(self.h, C.byref(sh)))
+
+    def autotune(self, steps: int = 2):
+        """Pick the fastest launch shape of each tunable kernel by timing
+        `steps` steps per candidate; the state is restored afterwards. Returns
+        (chosen shapes, {kernel field: {candidate: device us per step}})."""
+        sh = abi.SldShapes()
+        us = np.zeros(15)
+        self._rc(self.L.sld_autotune(self.h, int(steps), C.byref(sh), us.ctypes.data_as(_dp)))
+        table = {}
+        for k, (name, _) in enumerate(abi.SldShapes._fields_):
+            cands = abi.SHAPE_CANDIDATES[name]
+            table[name] = {c: float(us[3 * k + q]) for q, c in enumerate(cands)}
+        return {k: int(getattr(sh, k)) for k, _ in abi.SldShapes._fields_}, table
 
     def download(self) -> Dict[str, np.ndarray]:
         out = abi.AtomBuffers.empty(self.natom)
