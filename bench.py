@@ -222,6 +222,12 @@ def batched_vs_serial(steps: int, warmup: int):
     t0 = time.perf_counter()
     ob = b.step(warmup + steps, steps)
     tb = time.perf_counter() - t0
+    # the same steps replayed from one graph per step
+    b.set_graph_mode(2)
+    b.step(warmup + 2 * steps, 1)
+    t0 = time.perf_counter()
+    b.step(warmup + 2 * steps + 1, steps)
+    t_graph = time.perf_counter() - t0
     b.close()
     ts = 0.0
     equal = True
@@ -238,6 +244,7 @@ def batched_vs_serial(steps: int, warmup: int):
             "serial_atom_steps_per_s": natoms * steps / ts,
             "speedup": ts / tb, "observables_bitwise_equal": equal,
             "untuned_batched_atom_steps_per_s": natoms * steps / t_default,
+            "step_graph_batched_atom_steps_per_s": natoms * steps / t_graph,
             "autotune": {"default": default_shapes, "chosen": tuned, "us_per_step": table},
             "workload": "64 bcc Fe replicas, L^3 cubic cells, L uniform in {8..32}, 100-1400 K, "
                         "seed 100+r, Langevin lattice+spin, dt 1 fs"}
@@ -293,6 +300,8 @@ def main() -> int:
                     help="instead of the benchmark, measure NVE energy drift over this many steps")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="timed steps of the batched-vs-serial replica comparison (0: skip)")
+    ap.add_argument("--graph-steps", type=int, default=5,
+                    help="steps timed per graph mode after the timed region")
     ap.add_argument("--autotune-steps", type=int, default=2,
                     help="steps per candidate launch shape in the autotuner before timing (0: default shapes)")
     ap.add_argument("--adjoint-reps", type=int, default=10,
@@ -354,6 +363,18 @@ def main() -> int:
             "algorithmic_flop_per_launch": spec["flop"] * per_launch,
             "share_of_step": kms / step_ms if step_ms > 0 else None,
         }
+    # the same step issued eagerly (the default above), with the replayed
+    # core_dTe graph and as one replayed graph per step
+    graph_ms = {}
+    s1 = s0 + len([x for x in args.kernels.split(",") if x])
+    for mode in (0, 1, 2):
+        eng.set_graph_mode(mode)
+        eng.step(s1, 1)
+        t0 = time.perf_counter()
+        eng.step(s1 + 1, args.graph_steps)
+        graph_ms[str(mode)] = (time.perf_counter() - t0) / args.graph_steps * 1e3
+        s1 += 1 + args.graph_steps
+    eng.set_graph_mode(0)
     st = eng.download() if not args.no_cpu_baseline else None
     eng.close()
 
@@ -398,6 +419,7 @@ def main() -> int:
         "kernels": kern,
         "step_roofline": step_roof,
         "autotune": tune,
+        "ms_per_step_by_graph_mode": graph_ms,
         "batched": batched,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

@@ -200,6 +200,12 @@ typedef struct {
     int temperature_block; /* calculate_temperature: threads (32 per cell) per block; 64, 128 or 256 */
 } sld_shapes;
 int sld_get_shapes(sld_ctx* ctx, sld_shapes* out);
+/* How sld_step issues its work (no counterpart in the reference): 0 (default)
+ * eager launches; 1 core_dTe replayed from a captured graph; 2 each
+ * core() step replayed from a captured graph (one per cell-list buffer
+ * parity). Results are bitwise the same in every mode. The environment
+ * variable SLD_GRAPHS=0|1|2 sets the mode at creation. */
+int sld_set_graph_mode(sld_ctx* ctx, int mode);
 int sld_set_shapes(sld_ctx* ctx, const sld_shapes* in);
 /* Times each candidate shape of each kernel over `steps` full steps (after
  * one warm-up step) with HIP events, keeps the fastest, and restores the

@@ -61,11 +61,17 @@ struct sld_ctx {
     Tables tab{};
     Dev D{};
     hipStream_t stream = nullptr;
-    // core_dTe(dt) issues the same launches on every call with the same dt: it
-    // is captured once into a graph and replayed
-    bool use_graphs = true;
+    // graph_mode 0: eager launches (fastest on MI355X: the per-launch cost is
+    // on the device side); 1: core_dTe(dt) issues the same launches on every
+    // call with the same dt, so it is captured once into a graph and
+    // replayed; 2: the whole step is, one graph per cell-list buffer parity
+    int graph_mode = 0;
     hipGraphExec_t dte_exec = nullptr;
     double dte_exec_dt = 0.0;
+    hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    int* cl_first = nullptr;        // D.cl at creation (buffer parity of a step)
+    uint64_t step_npass = 0;        // noise passes of one step
+    bool capturing = false;
     int device = 0;
     LevelSet spin_fwd, spin_bwd, dte_fwd, dte_bwd;  // over all replicas
     // buffers
@@ -78,6 +84,7 @@ struct sld_ctx {
     double* d_einit = nullptr;               // initial_ave_energy and err
     double total_time = 0.0;
     uint64_t pass = 0;                       // noise pass counter
+    uint64_t pass_dev = 0;                   // value of the device counter D.pass_base
     bool nb_dirty = true;
     double last_ms = 0.0;
     // per-kernel event timing of one named kernel
@@ -299,6 +306,15 @@ int reduce_seg(sld_ctx* c, const int* d_seg, int nk, int out_off) {
     return SLD_OK;
 }
 
+// brings the device noise-pass counter up to the host's
+int sync_pass(sld_ctx* c) {
+    if (c->pass != c->pass_dev) {
+        LAUNCH(c, k_pass_advance, 1, 64, c->D, (int)(c->pass - c->pass_dev));
+        c->pass_dev = c->pass;
+    }
+    return SLD_OK;
+}
+
 int nbuild(sld_ctx* c, int write_rho) {
     const int w = c->shapes.nbuild_cells;
     LAUNCH(c, k_nbuild, nblk(c->D.C, w), 64 * w, c->tab, c->phys, c->D, write_rho);
@@ -332,12 +348,13 @@ int do_ds(sld_ctx* c, double dt) {
     double dth = dt / 2.0;
     for (int dir = 0; dir < 2; ++dir) {
         LevelSet& L = dir == 0 ? c->spin_fwd : c->spin_bwd;
-        if (c->phys.spinlang) LAUNCH(c, k_spin_noise, nblk(c->D.N, 256), 256, c->D, c->pass);
+        if (c->phys.spinlang)
+            LAUNCH(c, k_spin_noise, nblk(c->D.N, 256), 256, c->D, (int)(c->pass - c->pass_dev));
         for (int l = 0; l < L.nlev(); ++l) {
             int n = L.off[l + 1] - L.off[l];
             const int bs = c->shapes.spin_block;
             LAUNCH(c, k_spin_phase, nblk(n, bs), bs, L.d_cells + L.off[l], n, dir, dth, c->tab,
-                   c->phys, c->D, c->pass);
+                   c->phys, c->D);
         }
         c->pass++;
     }
@@ -348,7 +365,7 @@ int do_ds(sld_ctx* c, double dt) {
 // before and after centring
 int do_dp(sld_ctx* c, double dt) {
     int nb = nblk(c->D.C, 8);
-    LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, c->pass, c->d_part);
+    LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, (int)(c->pass - c->pass_dev), c->d_part);
     if (c->phys.lattlang) c->pass++;
     TRY(reduce_seg(c, c->d_cell8_seg, 3, 0));
     int na = nblk(c->D.N, 256);
@@ -377,7 +394,7 @@ int do_dTe_launches(sld_ctx* c, double dt);
 // core_dTe_CPU.cpp:47-217 through a graph of its launches when graphs are on
 // and no kernel is being timed
 int do_dTe(sld_ctx* c, double dt) {
-    if (!c->use_graphs || !c->timed_name.empty()) return do_dTe_launches(c, dt);
+    if (c->graph_mode < 1 || !c->timed_name.empty() || c->capturing) return do_dTe_launches(c, dt);
     if (c->dte_exec && c->dte_exec_dt != dt) {
         (void)hipGraphExecDestroy(c->dte_exec);
         c->dte_exec = nullptr;
@@ -545,6 +562,52 @@ int do_core(sld_ctx* c) {
         TRY(do_temperature(c));
         TRY(do_dTe(c, dt / 2.0));
     }
+    return sync_pass(c);
+}
+
+void drop_graphs(sld_ctx* c) {
+    if (c->dte_exec) (void)hipGraphExecDestroy(c->dte_exec);
+    c->dte_exec = nullptr;
+    for (auto& g : c->step_exec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+// one core() step: eager, or (graph_mode 2) through the graph of its launches.
+// Capturing runs the host logic of the step (buffer swap, pass counting)
+// once; a replay applies the same host effects.
+int run_step(sld_ctx* c) {
+    if (c->graph_mode < 2 || !c->timed_name.empty()) return do_core(c);
+    TRY(ensure_nb(c));  // a step starts with current lists
+    const int parity = c->D.cl == c->cl_first ? 0 : 1;
+    hipGraphExec_t& ex = c->step_exec[parity];
+    if (!ex) {
+        const uint64_t p0 = c->pass;
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        int rc = do_core(c);
+        c->capturing = false;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != SLD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_err(c, SLD_ERR_HIP, "capture of a step: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err(c, SLD_ERR_HIP, "instantiate step graph: %s", hipGetErrorString(e));
+        c->step_npass = c->pass - p0;
+        HIPCHK(c, hipGraphLaunch(ex, c->stream));  // host effects already applied by the capture
+        return SLD_OK;
+    }
+    HIPCHK(c, hipGraphLaunch(ex, c->stream));
+    std::swap(c->D.cl, c->d_cl2);
+    std::swap(c->D.cn, c->d_cn2);
+    c->pass += c->step_npass;
+    c->pass_dev = c->pass;
+    c->nb_dirty = false;
     return SLD_OK;
 }
 
@@ -698,7 +761,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     {
         const char* g = std::getenv("SLD_GRAPHS");
-        c->use_graphs = !(g && g[0] == '0');
+        if (g && g[0] >= '0' && g[0] <= '2') c->graph_mode = g[0] - '0';
     }
 
     // geometry of every replica (read_config.cpp:110-116, vec_utils_CPU.cpp:135-148,
@@ -796,6 +859,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
     TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
     TRY(dalloc(c, &D.zn, N));
+    TRY(dalloc(c, &D.pass_base, 1));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
     TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
@@ -914,6 +978,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     if (P0.eltemp) TRY(do_temperature(c));
     c->total_time = P0.start_time;
     c->pass = 0;
+    c->pass_dev = 0;
+    c->cl_first = c->D.cl;
     TRY(do_check(c, -1, c->d_obs));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return read_flags(c);
@@ -992,7 +1058,7 @@ int sld_select_replica(sld_ctx* c, int r) {
 int sld_destroy(sld_ctx* c) {
     if (!c) return SLD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->dte_exec) (void)hipGraphExecDestroy(c->dte_exec);
+    drop_graphs(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1005,8 +1071,8 @@ const char* sld_last_error(const sld_ctx* c) { return c ? c->err.c_str() : "null
 int sld_links(sld_ctx* c) { TRY(do_links(c)); return read_flags(c); }
 int sld_calculate_rho(sld_ctx* c) { TRY(nbuild(c, 1)); return read_flags(c); }
 int sld_calculate_force_energy(sld_ctx* c) { TRY(do_force(c)); return read_flags(c); }
-int sld_core_ds(sld_ctx* c, double dt) { TRY(do_ds(c, dt)); return read_flags(c); }
-int sld_core_dp(sld_ctx* c, double dt) { TRY(do_dp(c, dt)); return read_flags(c); }
+int sld_core_ds(sld_ctx* c, double dt) { TRY(do_ds(c, dt)); TRY(sync_pass(c)); return read_flags(c); }
+int sld_core_dp(sld_ctx* c, double dt) { TRY(do_dp(c, dt)); TRY(sync_pass(c)); return read_flags(c); }
 int sld_core_dr(sld_ctx* c, double dt) { TRY(do_dr(c, dt)); return read_flags(c); }
 
 int sld_core_dTe(sld_ctx* c, double dt) {
@@ -1049,7 +1115,7 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
     HIPCHK(c, hipEventRecord(e0, c->stream));
     int k = 0;
     for (int s = first_step; s < first_step + nsteps; ++s) {
-        TRY(do_core(c));
+        TRY(run_step(c));
         c->total_time += c->prm.step;
         if ((s + 1) % every == 0) {
             TRY(do_check(c, s, c->d_obs + (size_t)k * c->R));
@@ -1200,19 +1266,13 @@ bool shape_valid(const sld_shapes& s) {
     return true;
 }
 
-void drop_dte_graph(sld_ctx* c) {
-    if (c->dte_exec) {
-        (void)hipGraphExecDestroy(c->dte_exec);
-        c->dte_exec = nullptr;
-    }
-}
 
 // a device copy of every allocation and the host fields that a step changes
 struct Snapshot {
     void* buf = nullptr;
     Dev D{};
     int *cl2 = nullptr, *cn2 = nullptr;
-    uint64_t pass = 0;
+    uint64_t pass = 0, pass_dev = 0;
     double total_time = 0.0;
     bool nb_dirty = true;
 };
@@ -1232,6 +1292,7 @@ int snapshot_take(sld_ctx* c, Snapshot& s) {
     s.cl2 = c->d_cl2;
     s.cn2 = c->d_cn2;
     s.pass = c->pass;
+    s.pass_dev = c->pass_dev;
     s.total_time = c->total_time;
     s.nb_dirty = c->nb_dirty;
     return SLD_OK;
@@ -1253,6 +1314,7 @@ int snapshot_restore(sld_ctx* c, Snapshot& s) {
     c->d_cl2 = s.cl2;
     c->d_cn2 = s.cn2;
     c->pass = s.pass;
+    c->pass_dev = s.pass_dev;
     c->total_time = s.total_time;
     c->nb_dirty = s.nb_dirty;
     return SLD_OK;
@@ -1295,6 +1357,14 @@ int autotune_body(sld_ctx* c, int steps, double* us) {
 
 extern "C" {
 
+int sld_set_graph_mode(sld_ctx* c, int mode) {
+    if (!c) return SLD_ERR_ARG;
+    if (mode < 0 || mode > 2) return set_err(c, SLD_ERR_ARG, "graph mode %d is not 0, 1 or 2", mode);
+    if (mode != c->graph_mode) drop_graphs(c);
+    c->graph_mode = mode;
+    return SLD_OK;
+}
+
 int sld_get_shapes(sld_ctx* c, sld_shapes* out) {
     if (!c || !out) return SLD_ERR_ARG;
     *out = c->shapes;
@@ -1305,7 +1375,7 @@ int sld_set_shapes(sld_ctx* c, const sld_shapes* in) {
     if (!c || !in) return SLD_ERR_ARG;
     if (!shape_valid(*in)) return set_err(c, SLD_ERR_ARG, "launch shape outside the supported candidates");
     c->shapes = *in;
-    drop_dte_graph(c);
+    drop_graphs(c);
     return SLD_OK;
 }
 
@@ -1317,12 +1387,12 @@ int sld_autotune(sld_ctx* c, int steps, sld_shapes* best, double* us) {
     const sld_shapes before = c->shapes;
     Snapshot snap;
     TRY(snapshot_take(c, snap));
-    drop_dte_graph(c);
+    drop_graphs(c);
     int rc = autotune_body(c, steps, us);
     c->timed_name.clear();
     const std::string msg = c->err;
     int rc2 = snapshot_restore(c, snap);
-    drop_dte_graph(c);
+    drop_graphs(c);
     if (rc != SLD_OK) {
         c->shapes = before;
         c->err = msg;

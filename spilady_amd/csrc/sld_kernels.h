@@ -34,6 +34,7 @@ struct Dev {
     const int* crep;           // replica of each cell, -1 for padding [C]
     double4 *r, *p, *f, *s, *hh, *hl;
     double4* zn;               // Langevin normals of the current spin pass [N]
+    uint64_t* pass_base;       // noise passes consumed before the current step [1]
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
     double* stress;            // [6][N]: 11 22 33 12 23 31
     int* cell;                 // link cell of each atom (new_cell_index)
@@ -65,10 +66,11 @@ __global__ void k_relink(const int* cl_old, const int* cn_old, Dev D);
 __global__ void k_nbuild(Tables T, Phys P, Dev D, int write_rho);
 __global__ void k_fp(Tables T, Dev D);
 __global__ void k_force(Tables T, Phys P, Dev D);
-__global__ void k_spin_noise(Dev D, uint64_t pass);
+__global__ void k_spin_noise(Dev D, int pass_off);
+__global__ void k_pass_advance(Dev D, int n);
 __global__ void k_spin_phase(const int* cells, int ncell, int backward, double dt, Tables T, Phys P,
-                             Dev D, uint64_t pass);
-__global__ void k_dp_cells(double dt, Phys P, Dev D, uint64_t pass, double* partials);
+                             Dev D);
+__global__ void k_dp_cells(double dt, Phys P, Dev D, int pass_off, double* partials);
 __global__ void k_dp_center(Phys P, Dev D, const double* scal, double* partials);
 __global__ void k_dp_scale(Phys P, Dev D, const double* scal);
 __global__ void k_dr(double dt, Phys P, Dev D);

@@ -520,15 +520,23 @@ __device__ __forceinline__ double3 apply_rot(const Rot& r, double3 s) {
 
 // The Langevin normals of one spin pass for every atom (the counter-based
 // stream of normal3, keyed by replica seed and stream id, counted by the
-// atom's index in its replica and the pass).
-__global__ __launch_bounds__(256) void k_spin_noise(Dev D, uint64_t pass) {
+// atom's index in its replica and the pass). The pass number is the device
+// counter plus an offset fixed by the pass's place in the step, so that the
+// launches of a step are the same in every step.
+__global__ __launch_bounds__(256) void k_spin_noise(Dev D, int pass_off) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.N) return;
     const int rr = D.arep[i];
     if (rr < 0) return;
     const Rep& R = D.rep[rr];
+    const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
     const double3 z = normal3(R.seed, R.replica, i - R.atom0, pass);
     D.zn[i] = make_double4(z.x, z.y, z.z, 0.0);
+}
+
+// advances the device noise-pass counter by the passes a step consumed
+__global__ void k_pass_advance(Dev D, int n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.pass_base[0] += (uint64_t)n;
 }
 
 // Landau coefficients A..D of an atom: interpolation weights of its density,
@@ -554,8 +562,7 @@ __device__ __forceinline__ LanRows landau_rows(const Tables& T, double rho) {
 }
 
 __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell, int backward, double dt,
-                                                    Tables T, Phys P, Dev D, uint64_t pass) {
-    (void)pass;
+                                                    Tables T, Phys P, Dev D) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncell) return;
     const int c = cells[g];
@@ -706,7 +713,7 @@ __device__ __forceinline__ double3 dp_colmot(double3 p, int l, int n, double fac
 }
 
 // One half-wave (32 lanes) per link cell, one atom per lane.
-__global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, uint64_t pass,
+__global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int pass_off,
                                                   double* partials) {
     const int l = threadIdx.x & 31;
     const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
@@ -728,6 +735,7 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, uint
             const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
             double3 ff = make_double3(0.0, 0.0, 0.0);
             if (has) {
+                const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
                 ff = mul3(len, normal3(R.seed, R.replica, a - R.atom0, pass));
                 p = add3(p, mul3(dt, add3(d3(D.f[a]), ff)));
             }

@@ -98,6 +98,7 @@ def load(path: Optional[str] = None):
     L.sld_get_shapes.argtypes = [V, Sh]
     L.sld_set_shapes.argtypes = [V, Sh]
     L.sld_autotune.argtypes = [V, C.c_int, Sh, _dp]
+    L.sld_set_graph_mode.argtypes = [V, C.c_int]
     L.sld_model_nparams.argtypes = []
     L.sld_model_default_params.argtypes = [_dp]
     L.sld_model_set_params.argtypes = [V, _dp]
@@ -209,6 +210,10 @@ class Engine(abi.EngineLike):
         ms = C.c_double(0.0)
         self._rc(self.L.sld_kernel_timing(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def set_graph_mode(self, mode: int):
+        """0 eager launches, 1 core_dTe from a graph, 2 every step from a graph."""
+        self._rc(self.L.sld_set_graph_mode(self.h, int(mode)))
 
     def shapes(self) -> Dict[str, int]:
         """Launch shapes of the shape-independent kernels."""

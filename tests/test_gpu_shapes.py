@@ -1,4 +1,5 @@
-"""Launch shapes and the autotuner (sld_set_shapes, sld_autotune).
+"""Launch shapes, the autotuner (sld_set_shapes, sld_autotune) and the
+graph modes (sld_set_graph_mode).
 
 Every reduction inside the tunable kernels sums per wavefront, so the
 trajectory is the same bit for bit under any valid shapes; the autotuner
@@ -93,3 +94,38 @@ def test_autotune_leaves_the_trajectory_unchanged():
     assert tuned.total_time == plain.total_time
     tuned.close()
     plain.close()
+
+
+@pytest.mark.parametrize("print_every", [1, 3])
+def test_graph_modes_give_identical_trajectories(print_every):
+    """Eager launches, the replayed core_dTe graph and the replayed whole-step
+    graphs (both cell-list parities, several replays, observables computed
+    eagerly between replays) give the same trajectory bit for bit."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(8, 600.0, 43, thermostats=True, eltemp=True, localcolmot=True, print_every=print_every)
+    st = consistent_state(p, at)
+    runs = []
+    for mode in (0, 1, 2):
+        e = Engine(p, st)
+        e.set_graph_mode(mode)
+        obs = e.step(0, 5) + e.step(5, 7)
+        runs.append((obs, e.download(), e.cells()))
+        e.close()
+    obs0, a0, c0 = runs[0]
+    assert len(obs0) == 12 // print_every
+    for obs, a, c in runs[1:]:
+        assert len(obs) == len(obs0)
+        for k in range(len(obs0)):
+            assert _obs_equal(obs[k], obs0[k]), k
+        for f in FIELDS:
+            np.testing.assert_array_equal(a[f], a0[f], err_msg=f)
+        np.testing.assert_array_equal(c["te"], c0["te"])
+
+
+def test_graph_mode_rejects_unknown_modes():
+    from spilady_amd.engine import Engine, SldError
+    p, st = _full_physics(6)
+    e = Engine(p, st)
+    with pytest.raises(SldError):
+        e.set_graph_mode(3)
+    e.close()
