@@ -881,12 +881,9 @@ __device__ __forceinline__ void dte_A(int c, double h, const Phys& P, const Dev&
     D.Te[c] = ee_to_te(ee);
 }
 
-__device__ __forceinline__ void dte_B(int c, double h, const Phys& P, const Dev& D) {
-    double G = 0.0;
-    if (P.lattlang) G += D.Gel[c];
-    if (P.spinlang) G += D.Ges[c];
-    G *= h / kBoltz;
-    double T0 = D.Te[c];
+// RK4 of the electron-ion/spin relaxation over h (core_dTe_CPU.cpp:91-129);
+// G = (Gel + Ges) h / k_B
+__device__ __forceinline__ double dte_rk4(double T0, double G, double h) {
     double k1 = T0 * (exp(-G / ce_of(T0)) - 1.0) / h;
     double T1 = T0 + k1 * h / 2.0;
     double k2 = T1 * (exp(-G / ce_of(T1)) - 1.0) / h;
@@ -894,7 +891,18 @@ __device__ __forceinline__ void dte_B(int c, double h, const Phys& P, const Dev&
     double k3 = T2 * (exp(-G / ce_of(T2)) - 1.0) / h;
     double T3 = T0 + k3 * h;
     double k4 = T3 * (exp(-G / ce_of(T3)) - 1.0) / h;
-    D.Te[c] = T0 + h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+    return T0 + h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+}
+
+__device__ __forceinline__ double dte_G(int c, double h, const Phys& P, const Dev& D) {
+    double G = 0.0;
+    if (P.lattlang) G += D.Gel[c];
+    if (P.spinlang) G += D.Ges[c];
+    return G * (h / kBoltz);
+}
+
+__device__ __forceinline__ void dte_B(int c, double h, const Phys& P, const Dev& D) {
+    D.Te[c] = dte_rk4(D.Te[c], dte_G(c, h, P, D), h);
 }
 
 __global__ __launch_bounds__(256) void k_dte_ab(double h, Phys P, Dev D, double* partials) {
@@ -971,16 +979,6 @@ __global__ __launch_bounds__(256) void k_reduce2(const double* pa, int na, const
     }
 }
 
-// A(h) when E(Te) is already known (the previous A left it in Ee)
-__device__ __forceinline__ void dte_A_known(int c, double h, const Phys& P, const Dev& D, double ee) {
-    double dE = 0.0;
-    if (P.lattlang) dE += D.Gel[c] * D.Tl[c];
-    if (P.spinlang) dE += D.Ges[c] * D.TsL[c];
-    ee += h * dE;
-    D.Ee[c] = ee;
-    D.Te[c] = ee_to_te(ee);
-}
-
 // starting and final totals of a sub-step from n partial pairs (xp[0..n):
 // start, xp[n..2n): end), summed in a fixed order by every thread
 __device__ __forceinline__ double dte_factor(const double* xp, int n) {
@@ -994,19 +992,26 @@ __device__ __forceinline__ double dte_factor(const double* xp, int n) {
 
 // end of one sub-step and start of the next: renormalise (core_dTe_CPU.cpp:199-216),
 // B(h), A(h), then the next sub-step's A(h), B(h) and E(Te) as its C starts
+// In registers: the Te and Ee that the first A(h) leaves are overwritten by
+// the second before anything reads them, so only the final values are
+// computed and stored (same arithmetic as B, A, A, B applied in turn).
 __global__ __launch_bounds__(256) void k_dte_finish_ab(double h, Phys P, Dev D, const double* xp, int nxp,
                                                        double* partials) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (c < D.C) {
-        double f = dte_factor(xp, nxp);
-        double ee = D.Ee[c] * f;
-        D.Te[c] = ee_to_te(ee);
-        dte_B(c, h, P, D);
-        dte_A(c, h, P, D);
-        dte_A_known(c, h, P, D, D.Ee[c]);
-        dte_B(c, h, P, D);
-        ee = te_to_ee(D.Te[c]);
+        const double f = dte_factor(xp, nxp);
+        const double G = dte_G(c, h, P, D);
+        double dE = 0.0;
+        if (P.lattlang) dE += D.Gel[c] * D.Tl[c];
+        if (P.spinlang) dE += D.Ges[c] * D.TsL[c];
+        double T = ee_to_te(D.Ee[c] * f);
+        T = dte_rk4(T, G, h);                  // B(h) ends the sub-step
+        double ee = te_to_ee(T) + h * dE;      // A(h) ends it
+        ee += h * dE;                          // A(h) starts the next
+        T = dte_rk4(ee_to_te(ee), G, h);       // B(h)
+        ee = te_to_ee(T);                      // E(Te) as C starts
+        D.Te[c] = T;
         D.Ee[c] = ee;
         acc[0] = ee;
     }
