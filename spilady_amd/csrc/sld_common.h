@@ -15,6 +15,11 @@ constexpr int kRStride = 8;     // sf dsf pr dpr J dJ pad pad
 constexpr int kRhoStride = 16;  // F dF A B C D dA dB dC dD pad...
 
 enum RCol { R_SF = 0, R_DSF, R_PR, R_DPR, R_J, R_DJ };
+// Device r table, one 128-byte row per lookup index n holding the pair
+// (t[n-1], t[n]) that the interpolation at n reads, for each column: a pair
+// term's lookup touches one cache line instead of two 64-byte rows.
+constexpr int kPRowStride = 16;
+enum PCol { P_DSF = 0, P_DPR = 2, P_PR = 4, P_J = 6, P_DJ = 8, P_SF = 10 };
 enum RhoCol { H_F = 0, H_DF, H_A, H_B, H_C, H_D, H_DA, H_DB, H_DC, H_DD };
 
 // Fixed capacities; exceeding one sets an error flag that the host turns

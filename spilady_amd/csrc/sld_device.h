@@ -102,7 +102,7 @@ __device__ __forceinline__ int neigh_cell(const Box& b, int c, int slot) {
 //   r tables interleaved per entry: [sf dsf pr dpr J dJ - -]
 //   rho tables interleaved per entry: [F dF A B C D dA dB dC dD - ...]
 struct Tables {
-    const double* rtab;    // [NINPUT][8]
+    const double* prow;    // [NINPUT][16]: interpolation pairs of the r columns
     const double* rhotab;  // [NINPUT][16]
     double in_over_rmax, in_over_rhomax;
 };

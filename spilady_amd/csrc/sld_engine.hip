@@ -75,7 +75,7 @@ struct sld_ctx {
     int device = 0;
     LevelSet spin_fwd, spin_bwd, dte_fwd, dte_bwd;  // over all replicas
     // buffers
-    double *d_rtab = nullptr, *d_rhotab = nullptr;
+    double *d_prow = nullptr, *d_rhotab = nullptr;
     int *d_cl2 = nullptr, *d_cn2 = nullptr;  // second cell-list buffer
     double* d_part = nullptr;                // block partials
     double* d_scal = nullptr;                // reduced scalars
@@ -825,9 +825,9 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
 
     // tables (shared)
     std::shared_ptr<const HostTables> ht = shared_tables(P0.rcut_max);
-    TRY(upload_vec(c, &c->d_rtab, ht->rtab));
+    TRY(upload_vec(c, &c->d_prow, ht->prow));
     TRY(upload_vec(c, &c->d_rhotab, ht->rhotab));
-    c->tab.rtab = c->d_rtab;
+    c->tab.prow = c->d_prow;
     c->tab.rhotab = c->d_rhotab;
     c->tab.in_over_rmax = ht->in_over_rmax;
     c->tab.in_over_rhomax = ht->in_over_rhomax;

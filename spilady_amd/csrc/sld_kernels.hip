@@ -259,10 +259,11 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
                 int n0;
                 double ratio;
                 interp_index(r, T.in_over_rmax, n0, ratio);
-                if (r < P.rcut_pot) rho += lerp_tab(T.rtab, kRStride, R_SF, n0, ratio);
+                const double* q = T.prow + (size_t)n0 * kPRowStride;
+                if (r < P.rcut_pot) rho += q[P_SF] * ratio + q[P_SF + 1] * (1.0 - ratio);
                 if (inmag && pos2 < kMaxMag) {
                     nbm[pos2] = j;
-                    nbJ[pos2] = lerp_tab(T.rtab, kRStride, R_J, n0, ratio);
+                    nbJ[pos2] = q[P_J] * ratio + q[P_J + 1] * (1.0 - ratio);
                 }
             }
             nm += __popcll(m2);
@@ -325,14 +326,13 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
         int n0;
         double ratio;
         interp_index(r, T.in_over_rmax, n0, ratio);
-        const double* t0 = T.rtab + (size_t)(n0 - 1) * kRStride;
-        const double* t1 = t0 + kRStride;
+        const double* q = T.prow + (size_t)n0 * kPRowStride;
         const double wr = 1.0 - ratio;
         double dudr = 0.0;
         if (r < P.rcut_pot) {
-            double dsf = t0[R_DSF] * ratio + t1[R_DSF] * wr;
-            double dpr = t0[R_DPR] * ratio + t1[R_DPR] * wr;
-            double pr = t0[R_PR] * ratio + t1[R_PR] * wr;
+            double dsf = q[P_DSF] * ratio + q[P_DSF + 1] * wr;
+            double dpr = q[P_DPR] * ratio + q[P_DPR + 1] * wr;
+            double pr = q[P_PR] * ratio + q[P_PR + 1] * wr;
             dudr = (fpi + D.fp[j]) * dsf + dpr;
             pe += 0.5 * pr;
         }
@@ -340,8 +340,8 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
             double3 sj = d3(D.s[j]);
             double sdot = dot3(si, sj);
             double stimes = sqrt(si_sq * sq3(sj));
-            double J = t0[R_J] * ratio + t1[R_J] * wr;
-            double dJ = t0[R_DJ] * ratio + t1[R_DJ] * wr;
+            double J = q[P_J] * ratio + q[P_J + 1] * wr;
+            double dJ = q[P_DJ] * ratio + q[P_DJ + 1] * wr;
             dudr += -dJ * (sdot - stimes);
             double Jh = J / 2.0;
             me0 += Jh * stimes;

@@ -211,6 +211,16 @@ void build_tables(double rcut_max, HostTables& t) {
         if (a < b) th.emplace_back(work, a, b);
     }
     for (auto& x : th) x.join();
+    // interpolation rows of the device r table
+    t.prow.assign((size_t)kNInput * kPRowStride, 0.0);
+    const int cols[6][2] = {{P_DSF, R_DSF}, {P_DPR, R_DPR}, {P_PR, R_PR}, {P_J, R_J}, {P_DJ, R_DJ}, {P_SF, R_SF}};
+    for (int n = 1; n < kNInput; ++n) {
+        double* q = &t.prow[(size_t)n * kPRowStride];
+        for (const auto& cc : cols) {
+            q[cc[0]] = t.rtab[(size_t)(n - 1) * kRStride + cc[1]];
+            q[cc[0] + 1] = t.rtab[(size_t)n * kRStride + cc[1]];
+        }
+    }
 }
 
 std::shared_ptr<const HostTables> shared_tables(double rcut_max) {

@@ -9,6 +9,7 @@ namespace sld {
 struct HostTables {
     double rmax = 0, rhomax = 0, in_over_rmax = 0, in_over_rhomax = 0;
     std::vector<double> rtab;    // [kNInput][kRStride]
+    std::vector<double> prow;    // [kNInput][kPRowStride]: (rtab[n-1], rtab[n]) per column
     std::vector<double> rhotab;  // [kNInput][kRhoStride]
 };
 
