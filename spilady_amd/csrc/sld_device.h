@@ -107,6 +107,19 @@ struct Tables {
     double in_over_rmax, in_over_rhomax;
 };
 
+// Workgroups are dealt to the eight XCDs round-robin: block b runs on XCD
+// b % 8, and each XCD has its own L2. xcd_block maps a launched block to a
+// logical block so that every XCD works through one contiguous range of
+// logical blocks; spatially neighbouring atoms (whose positions, spins and
+// table rows a pair loop reads) then meet in the same L2. The grid is
+// launched rounded up to a multiple of 8; padding blocks get -1.
+__device__ __forceinline__ int xcd_block(int nlogical) {
+    const int b = blockIdx.x;
+    const int per = (nlogical + 7) >> 3;
+    const int L = (b & 7) * per + (b >> 3);
+    return L < nlogical ? L : -1;
+}
+
 __device__ __forceinline__ void interp_index(double x, double in_over_max, int& n, double& ratio) {
     double fn = x * in_over_max;
     n = (int)fn;

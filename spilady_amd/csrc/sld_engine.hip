@@ -146,6 +146,8 @@ int dalloc(sld_ctx* c, T** p, size_t n) {
 }
 
 inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
+// grid of a kernel that maps its blocks with xcd_block: a multiple of the 8 XCDs
+inline int xcd_grid(int nlogical) { return (nlogical + 7) / 8 * 8; }
 
 // Event pair around a launch of the timed kernel (see sld_time_kernel).
 hipEvent_t* timing_slot(sld_ctx* c, const char* name) {
@@ -317,7 +319,7 @@ int sync_pass(sld_ctx* c) {
 
 int nbuild(sld_ctx* c, int write_rho) {
     const int w = c->shapes.nbuild_cells;
-    LAUNCH(c, k_nbuild, nblk(c->D.C, w), 64 * w, c->tab, c->phys, c->D, write_rho);
+    LAUNCH(c, k_nbuild, xcd_grid(nblk(c->D.C, w)), 64 * w, c->tab, c->phys, c->D, write_rho);
     c->nb_dirty = false;
     return SLD_OK;
 }
@@ -338,8 +340,9 @@ int do_links(sld_ctx* c) {
 
 int do_force(sld_ctx* c) {
     TRY(ensure_nb(c));
-    LAUNCH(c, k_fp, nblk(c->D.N, 256), 256, c->tab, c->D);
-    LAUNCH(c, k_force, nblk(c->D.N, c->shapes.force_block), c->shapes.force_block, c->tab, c->phys, c->D);
+    LAUNCH(c, k_fp, xcd_grid(nblk(c->D.N, 256)), 256, c->tab, c->D);
+    LAUNCH(c, k_force, xcd_grid(nblk(c->D.N, c->shapes.force_block)), c->shapes.force_block, c->tab, c->phys,
+           c->D);
     return SLD_OK;
 }
 
@@ -384,7 +387,7 @@ int do_dr(sld_ctx* c, double dt) {
 int do_temperature(sld_ctx* c) {
     TRY(ensure_nb(c));
     const int bs = c->shapes.temperature_block;
-    LAUNCH(c, k_temperature, nblk(c->D.C, bs / 32), bs, c->tab, c->phys, c->D);
+    LAUNCH(c, k_temperature, xcd_grid(nblk(c->D.C, bs / 32)), bs, c->tab, c->phys, c->D);
     return SLD_OK;
 }
 
@@ -1474,7 +1477,7 @@ int model_rho(sld_ctx* c) {
     if (!m.ready) return set_err(c, SLD_ERR_STATE, "model parameters not set (sld_model_set_params)");
     TRY(ensure_nb(c));
     TRY(read_flags(c));
-    LAUNCH(c, k_model_rho, nblk(c->D.N, 256), 256, m.tab, c->phys, c->D, m.d_rho, m.d_fp);
+    LAUNCH(c, k_model_rho, xcd_grid(nblk(c->D.N, 256)), 256, m.tab, c->phys, c->D, m.d_rho, m.d_fp);
     return SLD_OK;
 }
 
@@ -1493,7 +1496,7 @@ int sld_model_eval(sld_ctx* c, double* energy, double* force, double* field) {
     c->ev_used = 0;
     TRY(model_rho(c));
     const int nb = nblk(c->D.N, 256);
-    LAUNCH(c, k_model_forward, nb, 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
+    LAUNCH(c, k_model_forward, xcd_grid(nb), 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
            (const int*)m.d_coff, m.d_F, m.d_H, m.d_part);
     LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)m.d_part, (const int*)c->d_atom_seg, 1, m.d_E, 0);
     if (energy)
@@ -1521,8 +1524,8 @@ int sld_model_vjp(sld_ctx* c, const double* g_energy, const double* g_force, con
     if (g_field) TRY(copy_any(c, m.d_gH, g_field, b3));
     else HIPCHK(c, hipMemsetAsync(m.d_gH, 0, b3, c->stream));
     const int nb = nblk(c->D.N, 256);
-    LAUNCH(c, k_model_q, nb, 256, m.tab, c->phys, c->D, (const int*)m.d_coff, (const double*)m.d_gF, m.d_Q);
-    LAUNCH(c, k_model_adjoint, nb, 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
+    LAUNCH(c, k_model_q, xcd_grid(nb), 256, m.tab, c->phys, c->D, (const int*)m.d_coff, (const double*)m.d_gF, m.d_Q);
+    LAUNCH(c, k_model_adjoint, xcd_grid(nb), 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
            (const double*)m.d_Q, (const int*)m.d_coff, (const double*)m.d_gE, (const double*)m.d_gF,
            (const double*)m.d_gH, m.d_part);
     LAUNCH(c, k_reduce, 1, 256, (const double*)m.d_part, nb, kNParam, m.d_grad);

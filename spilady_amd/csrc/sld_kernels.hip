@@ -177,7 +177,10 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
     __shared__ int s_cand[kNbWaves][kMaxCand];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int c = blockIdx.x * (blockDim.x >> 6) + w;
+    const int wpb = blockDim.x >> 6;
+    const int lb = xcd_block((D.C + wpb - 1) / wpb);
+    if (lb < 0) return;  // whole padding block, before any barrier
+    const int c = lb * wpb + w;
     const int rr = __builtin_amdgcn_readfirstlane(c < D.C ? D.crep[c] : -1);
     const bool valid = rr >= 0;
     Box b{};
@@ -286,7 +289,9 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
 // dF/drho plus the Landau derivative terms of one atom
 // (calculate_force_energy_CPU.cpp:98-107)
 __global__ __launch_bounds__(256) void k_fp(Tables T, Dev D) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;
+    int i = lb * blockDim.x + threadIdx.x;
     if (i >= D.N || D.arep[i] < 0) return;
     int n0;
     double ratio;
@@ -302,7 +307,9 @@ __global__ __launch_bounds__(256) void k_fp(Tables T, Dev D) {
 }
 
 __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;
+    int i = lb * blockDim.x + threadIdx.x;
     if (i >= D.N) return;
     const int rr = D.arep[i];
     if (rr < 0) return;
@@ -808,7 +815,9 @@ __global__ __launch_bounds__(256) void k_dr(double dt, Phys P, Dev D) {
 // per lane; the cell sums are lane reductions.
 __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
     const int l = threadIdx.x & 31;
-    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const int lb = xcd_block((D.C * 32 + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;
+    const int c = (lb * blockDim.x + threadIdx.x) >> 5;
     if (c >= D.C || D.crep[c] < 0) return;
     const int n = D.cn[c];
     const bool has = l < n;

@@ -23,7 +23,7 @@
 namespace sld {
 
 template <int NK>
-__device__ __forceinline__ void model_partials(const double (&v)[NK], double* partials) {
+__device__ __forceinline__ void model_partials(const double (&v)[NK], double* partials, int block) {
     __shared__ double sh[NK][256];
     const int t = threadIdx.x;
 #pragma unroll
@@ -38,7 +38,7 @@ __device__ __forceinline__ void model_partials(const double (&v)[NK], double* pa
     }
     if (t == 0) {
 #pragma unroll
-        for (int k = 0; k < NK; ++k) partials[(size_t)blockIdx.x * NK + k] = sh[k][0];
+        for (int k = 0; k < NK; ++k) partials[(size_t)block * NK + k] = sh[k][0];
     }
 }
 
@@ -46,7 +46,9 @@ __device__ __forceinline__ void model_partials(const double (&v)[NK], double* pa
 // calculate_force_energy_CPU.cpp:98-107; the Fe Landau coefficients do not
 // depend on rho)
 __global__ __launch_bounds__(256) void k_model_rho(ModelTab M, Phys P, Dev D, double* rho_out, double* fp_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;
+    const int i = lb * blockDim.x + threadIdx.x;
     if (i >= D.N) return;
     const int rr = D.arep[i];
     if (rr < 0) return;
@@ -75,7 +77,9 @@ __global__ __launch_bounds__(256) void k_model_rho(ModelTab M, Phys P, Dev D, do
 __global__ __launch_bounds__(256) void k_model_forward(ModelTab M, Phys P, Dev D, const double* rho,
                                                        const double* fp, const int* coff, double* F,
                                                        double* H, double* partials) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;  // whole padding block
+    const int i = lb * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     const int rr = i < D.N ? D.arep[i] : -1;
     if (rr >= 0) {
@@ -132,13 +136,15 @@ __global__ __launch_bounds__(256) void k_model_forward(ModelTab M, Phys P, Dev D
         H[3 * c] = h.x; H[3 * c + 1] = h.y; H[3 * c + 2] = h.z;
         acc[0] = e;
     }
-    model_partials<1>(acc, partials);
+    model_partials<1>(acc, partials, lb);
 }
 
 // adjoint, first pass: Q_i = sum_j (c_ij + c_ji) f'(r_ij)
 __global__ __launch_bounds__(256) void k_model_q(ModelTab M, Phys P, Dev D, const int* coff, const double* gF,
                                                  double* Q) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;
+    const int i = lb * blockDim.x + threadIdx.x;
     if (i >= D.N) return;
     const int rr = D.arep[i];
     if (rr < 0) return;
@@ -171,7 +177,9 @@ __global__ __launch_bounds__(256) void k_model_adjoint(ModelTab M, Phys P, Dev D
                                                        const double* fp, const double* Q, const int* coff,
                                                        const double* gE, const double* gF, const double* gH,
                                                        double* partials) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
+    if (lb < 0) return;  // whole padding block
+    const int i = lb * blockDim.x + threadIdx.x;
     double acc[kNParam];
 #pragma unroll
     for (int k = 0; k < kNParam; ++k) acc[k] = 0.0;
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(256) void k_model_adjoint(ModelTab M, Phys P, Dev D
         acc[P_LA + 2] += ge * s6 - 6.0 * s4 * gh_si;
         acc[P_LA + 3] += ge * s4 * s4 - 8.0 * s6 * gh_si;
     }
-    model_partials<kNParam>(acc, partials);
+    model_partials<kNParam>(acc, partials, lb);
 }
 
 }  // namespace sld
