@@ -75,7 +75,8 @@ typedef struct sld_atoms {
 
 /* One printed step: the enr-, tmp- and spn- columns
  * (check_energy_CPU.cpp:121-133, check_temperature_CPU.cpp:144-158,
- * check_spin_CPU.cpp:66-76) plus the virial. */
+ * check_spin_CPU.cpp:66-76), the virial, and the str- and prs- columns
+ * (check_stress_CPU.cpp:35-58, check_pressure_CPU.cpp:35-41) in GPa. */
 typedef struct sld_obs {
     int32_t step;
     int32_t pad;
@@ -84,6 +85,8 @@ typedef struct sld_obs {
     double tl, ts_r, ts_l, te;
     double ave_s[3], ave_s0, ave_m[3], ave_m0;
     double virial;
+    double ave_stress[6];      /* 11 22 33 12 23 31 */
+    double pressure;
 } sld_obs;
 
 /* Per-cell electron subsystem (struct.h:122-166); any pointer may be NULL. */

@@ -1153,6 +1153,15 @@ void orc_check(orc_ctx* o, int step, sld_obs* out) {
     ob.ave_m[0] = am.x; ob.ave_m[1] = am.y; ob.ave_m[2] = am.z;
     ob.ave_m0 = v_length(am);
     ob.virial = o->virial;
+    /* check_stress_CPU.cpp:35-58, check_pressure_CPU.cpp:35-41 */
+    double st[6] = {0e0, 0e0, 0e0, 0e0, 0e0, 0e0};
+    for (int i = 0; i < N; ++i)
+        for (int k = 0; k < 6; ++k) st[k] += o->stress[6 * i + k];
+    for (int k = 0; k < 6; ++k) ob.ave_stress[k] = st[k] * (160.217653e0 / N);
+    double density = (double)N / o->box_volume;
+    double tmp = 2e0 / 3e0 * sk / N;
+    ob.pressure = density * (tmp - o->virial / (3e0 * N));
+    ob.pressure *= 160.217653e0;
     if (out) *out = ob;
 }
 

@@ -52,6 +52,7 @@ class SldObs(C.Structure):
         ("tl", C.c_double), ("ts_r", C.c_double), ("ts_l", C.c_double), ("te", C.c_double),
         ("ave_s", C.c_double * 3), ("ave_s0", C.c_double), ("ave_m", C.c_double * 3),
         ("ave_m0", C.c_double), ("virial", C.c_double),
+        ("ave_stress", C.c_double * 6), ("pressure", C.c_double),
     ]
 
 
@@ -77,9 +78,10 @@ CELL_FIELDS = ("te", "ee", "tl", "ts_r", "ts_l", "gel", "ges", "sum_r_up", "sum_
 
 
 def obs_dict(ob: SldObs) -> Dict[str, float]:
-    d = {k: getattr(ob, k) for k, _ in SldObs._fields_ if k not in ("pad", "ave_s", "ave_m")}
+    d = {k: getattr(ob, k) for k, _ in SldObs._fields_ if k not in ("pad", "ave_s", "ave_m", "ave_stress")}
     d["ave_s"] = np.array(ob.ave_s[:])
     d["ave_m"] = np.array(ob.ave_m[:])
+    d["ave_stress"] = np.array(ob.ave_stress[:])
     return d
 
 

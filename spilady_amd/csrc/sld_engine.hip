@@ -462,7 +462,7 @@ int do_dTe_launches(sld_ctx* c, double dt) {
             (void)hipEventRecord(ev[1], c->stream);
             c->ev_launches[(ev - c->ev_pool.data()) / 2] = nlaunch;
         }
-        LAUNCH(c, k_reduce2, 1, 256, (const double*)c->d_part, nb, (const double*)(c->d_part + nb), npb,
+        LAUNCH(c, k_reduce2, 1, 1024, (const double*)c->d_part, nb, (const double*)(c->d_part + nb), npb,
                c->d_scal + 8);
         if (it + 1 < nloop)
             LAUNCH(c, k_dte_finish_ab, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8), 1, c->d_part);
@@ -485,7 +485,7 @@ __global__ void k_obs_final(const double* scal, const Rep* rep, int R, int eltem
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const double* sa = scal + (size_t)r * kScal + 16;
-    const double* sc = scal + (size_t)r * kScal + 32;
+    const double* sc = scal + (size_t)r * kScal + 36;
     const int N = rep[r].natom, C = rep[r].ncell;
     sld_obs o = {};
     o.step = step;
@@ -518,6 +518,12 @@ __global__ void k_obs_final(const double* scal, const Rep* rep, int R, int eltem
     o.ave_m[0] = -kElG * sx; o.ave_m[1] = -kElG * sy; o.ave_m[2] = -kElG * sz;
     o.ave_m0 = sqrt(o.ave_m[0] * o.ave_m[0] + o.ave_m[1] * o.ave_m[1] + o.ave_m[2] * o.ave_m[2]);
     o.virial = sa[11];
+    // check_stress_CPU.cpp:53-58 and check_pressure_CPU.cpp:35-41 (eV/A^3 -> GPa)
+    for (int k = 0; k < 6; ++k) o.ave_stress[k] = sa[12 + k] * (160.217653e0 / N);
+    const double density = (double)N / rep[r].box_volume;
+    const double tmp = 2e0 / 3e0 * sa[1] / N;
+    o.pressure = density * (tmp - o.virial / (3e0 * N));
+    o.pressure *= 160.217653e0;
     out[r] = o;
 }
 
@@ -536,15 +542,15 @@ int do_check(sld_ctx* c, int step, sld_obs* d_out) {
     if (c->phys.eltemp) {
         int nb = nblk(c->D.C, 256);
         LAUNCH(c, k_obs_cells, nb, 256, c->D, c->d_part);
-        TRY(reduce(c, nb, kObsCellK, c->d_scal + 32));
+        TRY(reduce(c, nb, kObsCellK, c->d_scal + 36));
     }
     LAUNCH(c, k_obs_final, nblk(c->R, 64), 64, (const double*)c->d_scal, (const Rep*)c->D.rep, c->R,
            c->phys.eltemp, c->prm.renormalize_energy, step, c->total_time, c->d_einit, d_out);
     if (c->phys.eltemp && c->prm.renormalize_energy) {
         int nb = nblk(c->D.C, 256);
         LAUNCH(c, k_renorm, nb, 256, c->D, (const double*)(c->d_einit + 1), c->d_part);
-        TRY(reduce(c, nb, 1, c->d_scal + 40));
-        LAUNCH(c, k_obs_set_te, 1, 64, (const double*)(c->d_scal + 40), c->D.C, d_out);
+        TRY(reduce(c, nb, 1, c->d_scal + 44));
+        LAUNCH(c, k_obs_set_te, 1, 64, (const double*)(c->d_scal + 44), c->D.C, d_out);
     }
     return SLD_OK;
 }

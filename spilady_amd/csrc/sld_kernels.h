@@ -104,7 +104,7 @@ __global__ void k_model_adjoint(ModelTab M, Phys P, Dev D, const double* rho, co
                                 const int* coff, const double* gE, const double* gF, const double* gH,
                                 double* partials);
 
-constexpr int kObsAtomK = 11;  // pe ke me me0 sx sy sz rup rdn lup ldn
+constexpr int kObsAtomK = 17;  // pe ke me me0 sx sy sz rup rdn lup ldn, then (vir) stress[6]
 constexpr int kObsCellK = 6;   // Ee sRup sRdn sLup sLdn Te
 
 }  // namespace sld

@@ -964,18 +964,19 @@ __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, 
 }
 
 // the two totals of a sub-step: out[0] = sum of the starting-energy partials,
-// out[1] = sum of the final-energy partials
-__global__ __launch_bounds__(256) void k_reduce2(const double* pa, int na, const double* pb, int nb,
-                                                 double* out) {
-    __shared__ double sh[2][256];
+// out[1] = sum of the final-energy partials; one block of 1024 threads (a
+// fixed order, independent of how the partials were produced)
+__global__ __launch_bounds__(1024) void k_reduce2(const double* pa, int na, const double* pb, int nb,
+                                                  double* out) {
+    __shared__ double sh[2][1024];
     const int t = threadIdx.x;
     double a = 0.0, e = 0.0;
-    for (int k = t; k < na; k += 256) a += pa[k];
-    for (int k = t; k < nb; k += 256) e += pb[k];
+    for (int k = t; k < na; k += 1024) a += pa[k];
+    for (int k = t; k < nb; k += 1024) e += pb[k];
     sh[0][t] = a;
     sh[1][t] = e;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if (t < s) {
             sh[0][t] += sh[0][t + s];
             sh[1][t] += sh[1][t + s];
@@ -1057,6 +1058,8 @@ __global__ __launch_bounds__(256) void k_obs_atoms(Tables T, Phys P, Dev D, doub
         double3 s = d3(D.s[a]);
         acc[4] = s.x; acc[5] = s.y; acc[6] = s.z;
         acc[11] = D.vir[a];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[12 + k] = D.stress[(size_t)k * D.N + a];
         if (scan_spin) {
             // check_temperature_CPU.cpp:64-77,97-128 (no eltemp)
             double3 H;

@@ -116,11 +116,28 @@ std::string numbered(const char* front, const std::string& body, int n_output) {
     return front + body + buf;
 }
 
-void append_logs(const std::string& body, const sld_obs& o, int eltemp) {
+// check_energy/temperature/stress/pressure/spin (check.cpp:31-47): one line
+// per printed step in each of enr-, tmp-, str-, prs- and spn-
+void append_logs(const std::string& body, const sld_obs& o, const sld_params& prm) {
+    const int eltemp = prm.eltemp;
     FILE* e = std::fopen(("enr-" + body + ".dat").c_str(), "a");
     FILE* t = std::fopen(("tmp-" + body + ".dat").c_str(), "a");
     FILE* s = std::fopen(("spn-" + body + ".dat").c_str(), "a");
-    if (!e || !t || !s) fail("cannot append to the enr-/tmp-/spn- files");
+    FILE* st = std::fopen(("str-" + body + ".dat").c_str(), "a");
+    FILE* pr = std::fopen(("prs-" + body + ".dat").c_str(), "a");
+    if (!e || !t || !s || !st || !pr) fail("cannot append to the enr-/tmp-/spn-/str-/prs- files");
+    // check_stress_CPU.cpp:68-81 and check_pressure_CPU.cpp:49-55: step, time,
+    // the box matrix, the density, then the stresses or the pressure
+    const double* b = prm.box;
+    const double density = (double)prm.natom / (b[0] * b[2] * b[5]);
+    const double* sv = o.ave_stress;
+    std::fprintf(st, "%d %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e\n",
+                 o.step, o.total_time, b[0], b[1], b[2], b[3], b[4], b[5], density, (sv[0] + sv[1] + sv[2]) / 3e0,
+                 sv[0], sv[1], sv[2], sv[3], sv[4], sv[5]);
+    std::fprintf(pr, "%d %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e\n", o.step, o.total_time, b[0], b[1],
+                 b[2], b[3], b[4], b[5], density, o.pressure);
+    std::fclose(st);
+    std::fclose(pr);
     std::fprintf(e, "%d %.15e %.15e %.15e %.15e %.15e", o.step, o.total_time, o.ave_pe, o.ave_ke, o.ave_me,
                  o.ave_me0);
     if (eltemp) std::fprintf(e, " %.15e", o.ave_ee);
@@ -145,6 +162,9 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
     std::fprintf(f, "%d %.15e SLDHL\n", a.n, t);
     std::fprintf(f, "%.15e %.15e %.15e\n%.15e %.15e %.15e\n", a.box[0], a.box[1], a.box[2], a.box[3], a.box[4],
                  a.box[5]);
+    // local_volume: box_volume / natom (calculate_force_energy_CPU.cpp:276;
+    // box_volume = d.xx d.yy d.zz, read_config.cpp:126)
+    const double lv = a.box[0] * a.box[2] * a.box[5] / a.n;
     for (int i = 0; i < a.n; ++i) {
         const double* r = &a.r[3 * i];
         const double* p = &a.p[3 * i];
@@ -160,7 +180,7 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
                      "%.15e %.15e\n",
                      i, r[0], r[1], r[2], p[0], p[1], p[2], fo[0], fo[1], fo[2], st[0], st[1], st[2], st[3], st[4],
                      st[5], a.rho[i], a.ke[i], a.pe[i], s[0], s[1], s[2], s0, hh[0], hh[0], hh[2], hl[0], hl[0],
-                     hl[2], a.me[i], a.me0[i], 0.0);
+                     hl[2], a.me[i], a.me0[i], lv);
     }
     std::fclose(f);
     if (!eltemp) return;
@@ -229,7 +249,7 @@ int main(int argc, char** argv) {
     // check(-1) and write(-1) (spilady.cpp:38-40)
     sld_obs o;
     if (sld_check(ctx, -1, &o)) fail(sld_last_error(ctx));
-    append_logs(body, o, prm.eltemp);
+    append_logs(body, o, prm);
     write_config(ctx, body, 0, a, prm.eltemp);
 
     // the runstep loop (spilady.cpp:43-59): advance to the next configuration
@@ -243,7 +263,7 @@ int main(int argc, char** argv) {
         int nobs = 0;
         if (sld_step(ctx, (int)s, n, obs.data(), &nobs)) fail(sld_last_error(ctx));
         for (int k = 0; k < nobs; ++k) {
-            append_logs(body, obs[k], prm.eltemp);
+            append_logs(body, obs[k], prm);
             std::printf("%d total time = %.6e dt = %.3e\n", obs[k].step, obs[k].total_time, prm.step);
         }
         s = next;

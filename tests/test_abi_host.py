@@ -31,7 +31,7 @@ def test_ctypes_structs_match_header_sizes():
     """Field layout of the plain-data structs: sizes fixed by the header."""
     assert C.sizeof(abi.SldParams) == 4 * 2 + 8 * 6 + 8 * 11 + 8 + 4 * 6 + 4 * 8
     assert C.sizeof(abi.SldAtoms) == 14 * 8
-    assert C.sizeof(abi.SldObs) == 8 + 8 + 8 * 5 + 8 * 4 + 8 * 8 + 8
+    assert C.sizeof(abi.SldObs) == 8 + 8 + 8 * 5 + 8 * 4 + 8 * 8 + 8 + 8 * 6 + 8
 
 
 def test_plan_matches_reference_groups():

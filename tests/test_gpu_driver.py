@@ -1,5 +1,5 @@
 """The C++ host driver (spilady_amd/spilady_hip, spilady.cpp's loop): its
-enr-/tmp-/spn- logs and con- files against the in-process engine started
+enr-/tmp-/spn-/str-/prs- logs and con- files against the in-process engine started
 from the same con- file."""
 import os
 import subprocess
@@ -55,6 +55,16 @@ def test_driver_logs_match_engine(tmp_path, eltemp):
         assert list(row_t[2:]) == [fmt(x) for x in want_t]
         want_s = list(o["ave_s"]) + [o["ave_s0"]] + list(o["ave_m"]) + [o["ave_m0"]]
         assert list(row_s[2:]) == [fmt(x) for x in want_s]
+    # str- and prs-: the box matrix, the density, the stresses or the pressure
+    strl = io.read_log(os.path.join(tmp, "str-drv.dat"))
+    prsl = io.read_log(os.path.join(tmp, "prs-drv.dat"))
+    assert strl.shape == (len(obs), 16) and prsl.shape == (len(obs), 10)
+    box = [float(x) for x in st.box]
+    dens = st.natom / (box[0] * box[2] * box[5])
+    for row_st, row_p, o in zip(strl, prsl, obs):
+        sv = list(o["ave_stress"])
+        assert list(row_st[2:]) == [fmt(x) for x in box + [dens, sum(sv[:3]) / 3e0] + sv]
+        assert list(row_p[2:]) == [fmt(x) for x in box + [dens, o["pressure"]]]
     # configurations after steps 3 and 6, and the final one (write(-2))
     names = sorted(f for f in os.listdir(tmp) if f.startswith("con-drv_"))
     assert names == ["con-drv_0000.dat", "con-drv_0001.dat", "con-drv_0002.dat", "con-drv_9999.dat"]
@@ -62,6 +72,8 @@ def test_driver_logs_match_engine(tmp_path, eltemp):
     a = e.download()
     for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
         np.testing.assert_array_equal(getattr(fin, k), np.vectorize(fmt)(a[k]), err_msg=k)
+    # local_volume = box_volume / natom (calculate_force_energy_CPU.cpp:276)
+    assert np.all(fin.local_volume == fmt(box[0] * box[2] * box[5] / st.natom))
     if eltemp:
         assert os.path.exists(os.path.join(tmp, "cel-drv_9999.dat"))
     e.close()

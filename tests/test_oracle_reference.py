@@ -106,8 +106,8 @@ def test_entry_points_match_reference(name):
 
 @pytest.mark.parametrize("name", ["step_nve.npz", "step_lang.npz"])
 def test_trajectory_matches_reference(name):
-    """nsteps of core() + check() (renormalizeEnergy on): the enr-/tmp-/spn-
-    columns as printed (%.15e) and the final state, bit for bit."""
+    """nsteps of core() + check() (renormalizeEnergy on): the enr-/tmp-/spn-/
+    str-/prs- columns as printed (%.15e) and the final state, bit for bit."""
     g = _load(name)
     v, p, o = _oracle_from_fixture(g)
     n = int(g["nsteps"])
@@ -119,6 +119,13 @@ def test_trajectory_matches_reference(name):
     np.testing.assert_array_equal(as_printed(enr), g["enr"][1:, 2:])
     np.testing.assert_array_equal(as_printed(tmp), g["tmp"][1:, 2:])
     np.testing.assert_array_equal(as_printed(spn), g["spn"][1:, 2:])
+    # str- and prs- (check_stress_CPU.cpp:68-81, check_pressure_CPU.cpp:49-55)
+    box = [float(x) for x in g["box"]]
+    dens = g["in_r"].shape[0] / (box[0] * box[2] * box[5])
+    st = np.array([box + [dens, sum(x["ave_stress"][:3]) / 3e0] + list(x["ave_stress"]) for x in obs])
+    pr = np.array([box + [dens, x["pressure"]] for x in obs])
+    np.testing.assert_array_equal(as_printed(st), g["str"][1:, 2:])
+    np.testing.assert_array_equal(as_printed(pr), g["prs"][1:, 2:])
     a = o.download()
     for k in ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0"):
         np.testing.assert_array_equal(a[k], g[f"final_{k}"], err_msg=k)
