@@ -59,7 +59,12 @@ typedef struct sld_params {
     int32_t eltemp;            /* control.h: per-cell electron temperature */
     int32_t localcolmot;       /* control.h: local collective motion (needs eltemp) */
     int32_t renormalize_energy;/* control.h: renormalizeEnergy (needs eltemp) */
-    int32_t reserved[8];
+    int32_t changestep;        /* control.h: the time step adapts after every step (scale_step_CPU.cpp:33-97);
+                                  needs a single replica */
+    int32_t reserved0;
+    double displace_limit;     /* A, largest displacement per step (default.h:218) */
+    double phi_limit;          /* rad, largest spin rotation per step (default.h:219) */
+    int32_t reserved[2];
 } sld_params;
 
 /* Host view of the per-atom state (struct.h:44-120, con- columns of
@@ -165,6 +170,13 @@ int sld_check(sld_ctx* ctx, int step, sld_obs* out);
  * obs (may be NULL) receives nrep records per printed step, printed-step
  * major (obs[k * nrep + r]); *nobs (may be NULL) the number of printed steps. */
 int sld_step(sld_ctx* ctx, int first_step, int nsteps, sld_obs* obs, int* nobs);
+/* scale_step (scale_step_CPU.cpp:33-97, called through scale(), scale.cpp): the
+ * largest displacement |p|max/m dt and spin rotation |H|max/hbar dt of the
+ * step just taken; dt *= 0.8 if either exceeds its limit, else dt *= 1.05.
+ * sld_step calls it after every step when changestep is on. */
+int sld_scale_step(sld_ctx* ctx);
+/* The current time step (s). */
+double sld_get_step(const sld_ctx* ctx);
 
 /* Device <-> host state of the selected replica (copy_atoms_from_GPU_to_CPU,
  * copy_CPU_btw_GPU.cu:141-150). Atoms are in the caller's original index order. */

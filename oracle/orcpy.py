@@ -77,6 +77,9 @@ def lib():
         L.orc_analytic.restype = C.c_double
         L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normal3.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint64, _dp]
+        L.orc_scale_step.argtypes = [C.c_void_p]
+        L.orc_get_step.argtypes = [C.c_void_p]
+        L.orc_get_step.restype = C.c_double
         L.orc_pair_v.argtypes = [C.c_double, _dp]
         L.orc_pair_v.restype = C.c_double
         L.orc_bridge_coeffs.argtypes = [_dp, _dp]
@@ -185,6 +188,13 @@ class Oracle(abi.EngineLike):
     @property
     def total_time(self) -> float:
         return self.L.orc_total_time(self.h)
+
+    @property
+    def dt(self) -> float:
+        return self.L.orc_get_step(self.h)
+
+    def scale_step(self):
+        self.L.orc_scale_step(self.h)
 
     def table(self, which: int, x: float) -> float:
         return self.L.orc_table(self.h, which, x)

@@ -150,6 +150,8 @@ void ref_core_dr(double dt) { Quiet q(false); core_dr(dt); }
 void ref_core_dTe(double dt) { Quiet q(false); core_dTe(dt); }
 void ref_calculate_temperature(void) { Quiet q(false); calculate_temperature(); }
 void ref_check(int cur) { Quiet q(true); check(cur); }
+// scale() (scale.cpp:31-44): with the shipped control.h, scale_step only
+void ref_scale(int cur) { Quiet q(true); scale(cur); }
 
 int ref_natom(void) { return natom; }
 int ref_ncells(void) { return ncells; }

@@ -27,7 +27,7 @@ REF_KEYS = ["OMP_threads", "out_body", "step", "interval_of_print_out",
             "interval_of_config_out", "in_config", "kappa_e", "start_time",
             "atmass", "temperature", "gamma_L_over_mass", "gamma_S_HL",
             "random_seed", "rcut_pot", "rcut_mag", "rcut_max",
-            "min_length_link_cell"]
+            "min_length_link_cell", "displace_limit", "phi_limit"]
 
 # Fixed switches of the reference build (shipped control.h).
 REF_SWITCHES = {"spinlang": 1, "lattlang": 1, "eltemp": 1, "localcolmot": 1,
@@ -55,6 +55,7 @@ class Reference:
         for nm in ("ref_core_ds", "ref_core_dp", "ref_core_dr", "ref_core_dTe"):
             getattr(L, nm).argtypes = [C.c_double]
         L.ref_check.argtypes = [C.c_int]
+        L.ref_scale.argtypes = [C.c_int]
         for nm in ("ref_step", "ref_total_time", "ref_virial", "ref_initial_ave_energy"):
             getattr(L, nm).restype = C.c_double
         L.ref_scalar.restype = C.c_double

@@ -1185,6 +1185,30 @@ void orc_core(orc_ctx* o, int step) {
 }
 
 /* spilady.cpp:43-57 with runstep: core, total_time += step, check */
+/* scale_step_CPU.cpp:33-97 (single thread): the largest |p|^2 and
+ * |Heff_H|^2 of the step just taken set the displacement and rotation
+ * estimates; dt shrinks by 0.8 if either exceeds its limit, else grows by 1.05 */
+void orc_scale_step(orc_ctx* o) {
+    double displace_max = 0e0, phi_max = 0e0;
+    for (int i = 0; i < o->natom; ++i) {
+        double p_sq = v_sq(o->p[i]);
+        if (p_sq > displace_max) displace_max = p_sq;
+    }
+    displace_max = sqrt(displace_max) / o->prm.atmass * o->prm.step;
+    for (int i = 0; i < o->natom; ++i) {
+        double omega_sq = v_sq(o->heff_h[i]);
+        if (omega_sq > phi_max) phi_max = omega_sq;
+    }
+    phi_max = sqrt(phi_max) / HBAR * o->prm.step;
+    int switch_lattice = displace_max > o->prm.displace_limit ? 1 : 0;
+    int switch_spin = phi_max > o->prm.phi_limit ? 1 : 0;
+    if (switch_spin + switch_lattice > 0) o->prm.step *= 0.80;
+    else o->prm.step *= 1.05;
+}
+
+double orc_get_step(const orc_ctx* o) { return o->prm.step; }
+
+/* spilady.cpp:43-57: core, time, check on printed steps, scale */
 int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs) {
     int k = 0;
     int every = o->prm.interval_of_print_out > 0 ? o->prm.interval_of_print_out : 1;
@@ -1195,6 +1219,7 @@ int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs) {
             orc_check(o, s, obs ? obs + k : NULL);
             ++k;
         }
+        if (o->prm.changestep) orc_scale_step(o);
     }
     return k;
 }

@@ -35,7 +35,11 @@ class SldParams(C.Structure):
         ("eltemp", C.c_int32),
         ("localcolmot", C.c_int32),
         ("renormalize_energy", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("changestep", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("displace_limit", C.c_double),
+        ("phi_limit", C.c_double),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -175,6 +179,10 @@ def params_from_variables(v: Dict[str, object], box, natom: int, replica: int = 
     p.eltemp = int(v.get("eltemp", 1))
     p.localcolmot = int(v.get("localcolmot", 1)) if p.eltemp else 0
     p.renormalize_energy = int(v.get("renormalizeEnergy", 1)) if p.eltemp else 0
+    # changestep (control.h) and its limits (default.h:218-219)
+    p.changestep = int(v.get("changestep", 0))
+    p.displace_limit = float(v.get("displace_limit", 0.01))
+    p.phi_limit = float(v.get("phi_limit", 2.0 * np.pi / 10.0))
     return p
 
 

@@ -397,7 +397,8 @@ int do_dTe_launches(sld_ctx* c, double dt);
 // core_dTe_CPU.cpp:47-217 through a graph of its launches when graphs are on
 // and no kernel is being timed
 int do_dTe(sld_ctx* c, double dt) {
-    if (c->graph_mode < 1 || !c->timed_name.empty() || c->capturing) return do_dTe_launches(c, dt);
+    if (c->graph_mode < 1 || !c->timed_name.empty() || c->capturing || c->prm.changestep)
+        return do_dTe_launches(c, dt);
     if (c->dte_exec && c->dte_exec_dt != dt) {
         (void)hipGraphExecDestroy(c->dte_exec);
         c->dte_exec = nullptr;
@@ -555,6 +556,24 @@ int do_check(sld_ctx* c, int step, sld_obs* d_out) {
     return SLD_OK;
 }
 
+// scale_step_CPU.cpp:33-97: shrink dt by 0.8 when the step just taken moved
+// an atom farther than displace_limit or turned a spin by more than
+// phi_limit, else grow it by 1.05
+int do_scale_step(sld_ctx* c) {
+    const int nb = nblk(c->D.N, 256);
+    LAUNCH(c, k_step_max, nb, 256, c->D, c->d_part);
+    LAUNCH(c, k_reduce_max2, 1, 256, (const double*)c->d_part, nb, c->d_scal + 48);
+    double mx[2] = {0.0, 0.0};
+    HIPCHK(c, hipMemcpyAsync(mx, c->d_scal + 48, sizeof(mx), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double displace_max = std::sqrt(mx[0]) / c->prm.atmass * c->prm.step;
+    const double phi_max = std::sqrt(mx[1]) / kHbar * c->prm.step;
+    const int sw = (displace_max > c->prm.displace_limit ? 1 : 0) + (phi_max > c->prm.phi_limit ? 1 : 0);
+    if (sw > 0) c->prm.step *= 0.80;
+    else c->prm.step *= 1.05;
+    return SLD_OK;
+}
+
 // core.cpp:31-76 (SLDHL)
 int do_core(sld_ctx* c) {
     double dt = c->prm.step;
@@ -587,7 +606,7 @@ void drop_graphs(sld_ctx* c) {
 // Capturing runs the host logic of the step (buffer swap, pass counting)
 // once; a replay applies the same host effects.
 int run_step(sld_ctx* c) {
-    if (c->graph_mode < 2 || !c->timed_name.empty()) return do_core(c);
+    if (c->graph_mode < 2 || !c->timed_name.empty() || c->prm.changestep) return do_core(c);
     TRY(ensure_nb(c));  // a step starts with current lists
     const int parity = c->D.cl == c->cl_first ? 0 : 1;
     hipGraphExec_t& ex = c->step_exec[parity];
@@ -733,7 +752,7 @@ bool same_shared(const sld_params& a, const sld_params& b) {
            a.min_length_link_cell == b.min_length_link_cell && a.start_time == b.start_time &&
            a.interval_of_print_out == b.interval_of_print_out && a.spinlang == b.spinlang &&
            a.lattlang == b.lattlang && a.eltemp == b.eltemp && a.localcolmot == b.localcolmot &&
-           a.renormalize_energy == b.renormalize_energy;
+           a.renormalize_energy == b.renormalize_energy && a.changestep == b.changestep;
 }
 
 int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms* init, int compute_forces) {
@@ -763,6 +782,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     const sld_params& P0 = c->prm;
     if (nrep > 1 && P0.eltemp)
         return set_err(c, SLD_ERR_ARG, "the electron-temperature model runs on single-system contexts only");
+    if (nrep > 1 && P0.changestep)
+        return set_err(c, SLD_ERR_ARG, "changestep runs on single-system contexts only (one time step per context)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_err(c, SLD_ERR_NOGPU, "no HIP device");
@@ -1130,6 +1151,8 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
             TRY(do_check(c, s, c->d_obs + (size_t)k * c->R));
             ++k;
         }
+        // scale() (spilady.cpp:50-57, scale.cpp:31-44)
+        if (c->prm.changestep) TRY(do_scale_step(c));
     }
     LAUNCH(c, k_check_nonfinite, nblk(c->D.N, 256), 256, c->D);
     HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -1246,6 +1269,14 @@ int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
 }
 
 double sld_total_time(const sld_ctx* c) { return c->total_time; }
+
+int sld_scale_step(sld_ctx* c) {
+    if (!c) return SLD_ERR_ARG;
+    TRY(do_scale_step(c));
+    return read_flags(c);
+}
+
+double sld_get_step(const sld_ctx* c) { return c ? c->prm.step : 0.0; }
 
 // ---------------------------------------------------------------------------
 // launch shapes and the autotuner

@@ -1042,6 +1042,59 @@ __global__ __launch_bounds__(256) void k_dte_finish(double h, Phys P, Dev D, con
 
 
 // ---------------------------------------------------------------------------
+// scale_step (scale_step_CPU.cpp:35-70): the largest |p|^2 and |Heff_H|^2
+
+__global__ __launch_bounds__(256) void k_step_max(Dev D, double* partials) {
+    __shared__ double sh[2][256];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = threadIdx.x;
+    double pm = 0.0, hm = 0.0;
+    if (a < D.N && D.arep[a] >= 0) {
+        pm = sq3(d3(D.p[a]));
+        hm = sq3(d3(D.hh[a]));
+    }
+    sh[0][t] = pm;
+    sh[1][t] = hm;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            sh[0][t] = fmax(sh[0][t], sh[0][t + s]);
+            sh[1][t] = fmax(sh[1][t], sh[1][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partials[2 * blockIdx.x] = sh[0][0];
+        partials[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
+// out[0..1] = the largest of the pair partials (exact in any order)
+__global__ __launch_bounds__(256) void k_reduce_max2(const double* partials, int nblk, double* out) {
+    __shared__ double sh[2][256];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int k = t; k < nblk; k += 256) {
+        a = fmax(a, partials[2 * k]);
+        b = fmax(b, partials[2 * k + 1]);
+    }
+    sh[0][t] = a;
+    sh[1][t] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            sh[0][t] = fmax(sh[0][t], sh[0][t + s]);
+            sh[1][t] = fmax(sh[1][t], sh[1][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // observables (check_energy_CPU.cpp, check_temperature_CPU.cpp, check_spin_CPU.cpp)
 
 __global__ __launch_bounds__(256) void k_obs_atoms(Tables T, Phys P, Dev D, double* partials,

@@ -48,7 +48,8 @@ struct Vars {
         {"gamma_L_over_mass", "6e11"}, {"gamma_S_HL", "5.88e13"}, {"random_seed", "1234"},
         {"rcut_pot", "4.1"}, {"rcut_mag", "3.75"}, {"rcut_max", "4.1"}, {"min_length_link_cell", "4.11"},
         {"spinlang", "1"}, {"lattlang", "1"}, {"eltemp", "1"}, {"localcolmot", "1"},
-        {"renormalizeEnergy", "1"}};
+        {"renormalizeEnergy", "1"}, {"runstep", "1"}, {"changestep", "0"},
+        {"total_production_time", "1e-12"}, {"displace_limit", "0.01"}, {"phi_limit", "0.628319"}};
     void read(const char* path) {
         std::ifstream in(path);
         if (!in) fail(std::string("cannot open ") + path);
@@ -236,6 +237,11 @@ int main(int argc, char** argv) {
     prm.eltemp = (int)v.i("eltemp");
     prm.localcolmot = prm.eltemp ? (int)v.i("localcolmot") : 0;
     prm.renormalize_energy = prm.eltemp ? (int)v.i("renormalizeEnergy") : 0;
+    prm.changestep = (int)v.i("changestep");
+    prm.displace_limit = v.d("displace_limit");
+    prm.phi_limit = v.d("phi_limit");
+    const bool runstep = v.i("runstep") != 0;
+    const double total_production_time = v.d("total_production_time");
     const std::string body = v.s("out_body");
     const long long nsteps = v.i("no_of_production_steps");
     const int every_config = (int)std::max(1LL, v.i("interval_of_config_out"));
@@ -252,9 +258,28 @@ int main(int argc, char** argv) {
     append_logs(body, o, prm);
     write_config(ctx, body, 0, a, prm.eltemp);
 
+    std::vector<sld_obs> obs;
+    if (!runstep) {
+        // the time loop (spilady.cpp:46-57): one step per call while
+        // total_time <= total_production_time; dt may change every step
+        // (changestep)
+        obs.resize(1);
+        long long s = 0;
+        while (sld_total_time(ctx) <= total_production_time + 1e-18) {
+            int nobs = 0;
+            if (sld_step(ctx, (int)s, 1, obs.data(), &nobs)) fail(sld_last_error(ctx));
+            if (nobs > 0) append_logs(body, obs[0], prm);
+            std::printf("%lld total time = %.6e dt = %.3e\n", s, sld_total_time(ctx), sld_get_step(ctx));
+            ++s;
+            if (s % every_config == 0) write_config(ctx, body, (int)((s - 1) / every_config + 1), a, prm.eltemp);
+        }
+        write_config(ctx, body, 9999, a, prm.eltemp);  // write(-2)
+        sld_destroy(ctx);
+        std::printf("SPILADY (libspilady_hip) ends\n");
+        return 0;
+    }
     // the runstep loop (spilady.cpp:43-59): advance to the next configuration
     // output in one call; every printed step comes back as one record
-    std::vector<sld_obs> obs;
     for (long long s = 0; s < nsteps;) {
         long long next = (s / every_config + 1) * every_config;
         if (next > nsteps) next = nsteps;

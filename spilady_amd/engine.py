@@ -91,6 +91,9 @@ def load(path: Optional[str] = None):
     L.sld_get_cell_lists.argtypes = [V, C.c_int, _ip, _ip]
     L.sld_total_time.argtypes = [V]
     L.sld_total_time.restype = C.c_double
+    L.sld_scale_step.argtypes = [V]
+    L.sld_get_step.argtypes = [V]
+    L.sld_get_step.restype = C.c_double
     L.sld_last_step_timing.argtypes = [V, _dp]
     L.sld_time_kernel.argtypes = [V, C.c_char_p]
     L.sld_kernel_timing.argtypes = [V, _ip, _dp]
@@ -271,6 +274,15 @@ class Engine(abi.EngineLike):
     @property
     def total_time(self) -> float:
         return self.L.sld_total_time(self.h)
+
+    @property
+    def dt(self) -> float:
+        """The current time step (changes under changestep)."""
+        return self.L.sld_get_step(self.h)
+
+    def scale_step(self):
+        """scale_step (scale_step_CPU.cpp:33-97)."""
+        self._rc(self.L.sld_scale_step(self.h))
 
 
 class BatchEngine(Engine):

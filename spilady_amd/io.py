@@ -50,6 +50,12 @@ VARIABLE_DEFAULTS: Dict[str, object] = {
     "eltemp": 1,
     "localcolmot": 1,
     "renormalizeEnergy": 1,
+    "runstep": 1,              # 0: run while total_time <= total_production_time
+    "changestep": 0,           # adapt dt after every step (scale_step_CPU.cpp)
+    # time loop and changestep limits (default.h:218-219, read_variables.cpp:79,238-241)
+    "total_production_time": 1e-12,
+    "displace_limit": 0.01,
+    "phi_limit": 0.628319,
 }
 
 

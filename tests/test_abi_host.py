@@ -96,7 +96,8 @@ def _batch_args(params_list):
 
 def test_batch_rejects_mismatched_or_eltemp_replicas():
     """Argument checks of sld_create_batch that run before any device call:
-    replicas must share dt and the switches, and eltemp needs one replica."""
+    replicas must share dt and the switches, and eltemp and changestep need
+    one replica."""
     lib = engine.load()
     at = configs.make_bcc_fe(6, 300.0, 1)
     v = configs.fe_variables(300.0, 1, thermostats=True)
@@ -115,4 +116,12 @@ def test_batch_rejects_mismatched_or_eltemp_replicas():
     h = C.c_void_p()
     rc = lib.sld_create_batch(2, pv, av, 0, C.byref(h))
     assert rc == 1 and b"single-system" in lib.sld_last_error(h)
+    lib.sld_destroy(h)
+    # changestep adapts one time step per context
+    c0 = abi.params_from_variables(dict(v, changestep=1), at.box, at.natom)
+    c1 = abi.params_from_variables(dict(v, changestep=1), at.box, at.natom, replica=1)
+    pv, av, bufs = _batch_args([c0, c1])
+    h = C.c_void_p()
+    rc = lib.sld_create_batch(2, pv, av, 0, C.byref(h))
+    assert rc == 1 and b"changestep" in lib.sld_last_error(h)
     lib.sld_destroy(h)

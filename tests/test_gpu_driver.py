@@ -16,7 +16,7 @@ DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
                       "spilady_hip")
 
 
-def _setup(tmp, eltemp):
+def _setup(tmp, eltemp, **extra):
     v, p, at = fe_params(8, 600.0, 21, thermostats=True, eltemp=eltemp, localcolmot=eltemp)
     st = consistent_state(p, at)
     state = io.AtomState(box=np.asarray(at.box), r=st["r"], p=st["p"], f=st["f"], s=st["s"],
@@ -25,6 +25,7 @@ def _setup(tmp, eltemp):
     io.write_con(os.path.join(tmp, "con-start.dat"), state)
     v = dict(v, in_config="con-start.dat", out_body="drv", no_of_production_steps=7,
              interval_of_print_out=2, interval_of_config_out=3)
+    v.update(extra)
     io.write_variables(os.path.join(tmp, "variables.in"), v)
     return v
 
@@ -77,3 +78,37 @@ def test_driver_logs_match_engine(tmp_path, eltemp):
     if eltemp:
         assert os.path.exists(os.path.join(tmp, "cel-drv_9999.dat"))
     e.close()
+
+
+def test_driver_time_loop_with_changestep(tmp_path):
+    """The shipped control.h loop: runstep off (run while total_time <=
+    total_production_time) with changestep on; the driver's step count, times
+    and energies against the in-process engine stepped the same way."""
+    from spilady_amd import abi
+    from spilady_amd.engine import Engine
+    tmp = str(tmp_path)
+    _setup(tmp, True, runstep=0, changestep=1, total_production_time=6e-15, displace_limit=0.01,
+           phi_limit=0.628319, interval_of_print_out=1)
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    st = io.read_con(os.path.join(tmp, "con-start.dat"))
+    v = io.read_variables(os.path.join(tmp, "variables.in"))
+    p = abi.params_from_variables(v, st.box, st.natom)
+    assert p.changestep == 1
+    e = Engine(p, st, compute_initial_forces=False)
+    obs = [e.check(-1)]
+    dts = [e.dt]
+    s = 0
+    while e.total_time <= 6e-15 + 1e-18:
+        obs += e.step(s, 1)
+        dts.append(e.dt)
+        s += 1
+    e.close()
+    fmt = lambda x: float("%.15e" % x)  # noqa: E731
+    enr = io.read_log(os.path.join(tmp, "enr-drv.dat"))
+    assert enr.shape[0] == len(obs)
+    assert len(set(dts)) > 2                          # the step did change
+    for row, o in zip(enr, obs):
+        assert int(row[0]) == o["step"]
+        assert row[1] == fmt(o["total_time"])
+        assert list(row[2:6]) == [fmt(o[k]) for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0")]

@@ -180,3 +180,30 @@ def test_core_dte_graph_replays_equal_eager_launches(monkeypatch):
         np.testing.assert_array_equal(engines[0].download()["p"], engines[1].download()["p"])
     for e in engines:
         e.close()
+
+
+def test_changestep_matches_oracle():
+    """changestep: 16 steps of core(), check() and scale_step() with
+    thermostats and the electron temperature (renormalisation off); the time
+    step sequence exactly, observables and final state at the tolerance."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(8, 600.0, 5, thermostats=True, eltemp=True, localcolmot=True)
+    p.changestep = 1
+    p.displace_limit = 0.01
+    p.phi_limit = 0.628319
+    st = consistent_state(p, at)
+    eng = Engine(p, st, compute_initial_forces=False)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_PHILOX)
+    dts_e, dts_o = [], []
+    for s in range(16):
+        x = orc.step(s, 1)[0]
+        y = eng.step(s, 1)[0]
+        dts_o.append(orc.dt)
+        dts_e.append(eng.dt)
+        for k in ("ave_pe", "ave_ke", "ave_me", "tl", "ts_r", "te", "pressure"):
+            assert abs(x[k] - y[k]) <= TOL * max(abs(x[k]), 1e-300), (s, k, x[k], y[k])
+    assert dts_e == dts_o
+    steps = np.array(dts_e)
+    assert np.sum(steps[1:] < steps[:-1]) >= 1 and np.sum(steps[1:] > steps[:-1]) >= 1
+    _compare(eng, orc, "changestep", cells=True)
+    eng.close(); orc.close()

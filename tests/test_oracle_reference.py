@@ -163,3 +163,42 @@ def test_live_reference_agrees_hot_lattice():
     np.testing.assert_array_equal(counts_r, counts_o)
     np.testing.assert_array_equal(lists_r, lists_o)
     o.close()
+
+
+@pytest.mark.skipif(not refpy.available(), reason="compiled reference not built (oracle/Makefile.ref)")
+def test_live_reference_changestep():
+    """changestep (scale_step_CPU.cpp:33-97): 16 steps of core() + scale()
+    with the reference's limits (0.01 A, 2 pi / 10 rad), where the time step
+    shrinks by 0.8 three times and grows by 1.05 otherwise; the dt sequence
+    and the final state bit for bit. check() is left out: the reference build
+    renormalises the electron energy in it, which this small system does not
+    survive."""
+    at = configs.make_bcc_fe(8, 600.0, 5, tilt_max_deg=10.0)
+    v = configs.fe_variables(600.0, 5, thermostats=True, eltemp=True, localcolmot=True, renormalize=False)
+    v["displace_limit"] = 0.01
+    v["phi_limit"] = 0.628319
+    st = refpy.consistent_start(v, at)
+    d = tempfile.mkdtemp()
+    ref = refpy.Reference()
+    ref.init_memory(d, v, st)
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), st.box, st.natom)
+    p.renormalize_energy = 0
+    p.changestep = 1
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    dt_ref, dt_orc = [], []
+    for cur in range(16):
+        ref.run(cur, 1, False)
+        ref.lib.ref_scale(cur)
+        dt_ref.append(ref.ref_step())
+        o.core(cur)
+        o.scale_step()
+        dt_orc.append(o.dt)
+    a = ref.atoms()
+    ref.free()
+    b = o.download()
+    o.close()
+    assert dt_ref == dt_orc
+    steps = np.array(dt_ref)
+    assert np.sum(steps[1:] < steps[:-1]) >= 2 and np.sum(steps[1:] > steps[:-1]) >= 2
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
