@@ -64,7 +64,9 @@ typedef struct sld_params {
     int32_t reserved0;
     double displace_limit;     /* A, largest displacement per step (default.h:218) */
     double phi_limit;          /* rad, largest spin rotation per step (default.h:219) */
-    int32_t reserved[2];
+    int32_t extfield;          /* control.h: per-atom external field (external_field_CPU.cpp), zero
+                                  until set with sld_set_external */
+    int32_t extforce;          /* control.h: per-atom external force (external_force_CPU.cpp), same */
 } sld_params;
 
 /* Host view of the per-atom state (struct.h:44-120, con- columns of
@@ -177,6 +179,17 @@ int sld_step(sld_ctx* ctx, int first_step, int nsteps, sld_obs* obs, int* nobs);
 int sld_scale_step(sld_ctx* ctx);
 /* The current time step (s). */
 double sld_get_step(const sld_ctx* ctx);
+/* The per-atom external field Hext (eV, i.e. -g muB B; external_field_CPU.cpp:
+ * 38-53) and external force fext (eV/A; external_force_CPU.cpp:38-50) of the
+ * selected replica, [natom][3] each in atom order; NULL leaves one unchanged.
+ * Needs the matching switch in sld_params. Hext starts every spin field sum
+ * (calculate_spin_CPU.cpp:35-39) and enters me as -s.Hext
+ * (calculate_force_energy_CPU.cpp:199-201); core_dp adds fext to f in place
+ * before the momentum update (core_dp_CPU.cpp:69-71,130-132). The reference
+ * reads the field before its initial calculate_temperature (initial.cpp:58-
+ * 106): with eltemp, call sld_calculate_temperature after setting it at the
+ * start of a run. */
+int sld_set_external(sld_ctx* ctx, const double* hext, const double* fext);
 
 /* Device <-> host state of the selected replica (copy_atoms_from_GPU_to_CPU,
  * copy_CPU_btw_GPU.cu:141-150). Atoms are in the caller's original index order. */

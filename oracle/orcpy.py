@@ -78,6 +78,7 @@ def lib():
         L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normal3.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint64, _dp]
         L.orc_scale_step.argtypes = [C.c_void_p]
+        L.orc_set_external.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_get_step.argtypes = [C.c_void_p]
         L.orc_get_step.restype = C.c_double
         L.orc_pair_v.argtypes = [C.c_double, _dp]
@@ -195,6 +196,13 @@ class Oracle(abi.EngineLike):
 
     def scale_step(self):
         self.L.orc_scale_step(self.h)
+
+    def set_external(self, hext=None, fext=None):
+        """Per-atom external field (eV) and force (eV/A), [natom][3]."""
+        h = None if hext is None else np.ascontiguousarray(hext, dtype=np.float64)
+        f = None if fext is None else np.ascontiguousarray(fext, dtype=np.float64)
+        self.L.orc_set_external(self.h, None if h is None else h.ctypes.data_as(_dp),
+                                None if f is None else f.ctypes.data_as(_dp))
 
     def table(self, which: int, x: float) -> float:
         return self.L.orc_table(self.h, which, x)

@@ -116,6 +116,13 @@ int ref_init_memory(const char* dir, int n, const double* dbox, const double* r,
         a->m = vec_times(-el_g, a->s);
         a->m0 = vec_length(a->m);
     }
+    // initial.cpp:58-63: extfield.in / extforce.in of the run directory
+#ifdef extfield
+    external_field(-1);
+#endif
+#ifdef extforce
+    external_force(-1);
+#endif
     map_cells();
     initial_links();
     allocate_cells();

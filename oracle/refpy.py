@@ -34,8 +34,16 @@ REF_SWITCHES = {"spinlang": 1, "lattlang": 1, "eltemp": 1, "localcolmot": 1,
                 "renormalizeEnergy": 1}
 
 
+# the same sources built with -Dextfield -Dextforce (Makefile.ref OUT=_ref/ext)
+REF_LIB_EXT = os.path.join(HERE, "_ref", "ext", "libspilady_ref.so")
+
+
 def available() -> bool:
     return os.path.exists(REF_LIB)
+
+
+def available_ext() -> bool:
+    return os.path.exists(REF_LIB_EXT)
 
 
 def _ptr(a: np.ndarray):

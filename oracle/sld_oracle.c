@@ -44,6 +44,7 @@ struct orc_ctx {
     double* tab[NTAB];
     /* atoms */
     vec *r, *p, *f, *s, *heff_h, *heff_l, *heff_hc;
+    vec *hext, *fext;          /* external field (eV) and force (eV/A), extfield/extforce */
     double *s0, *rho, *pe, *ke, *me, *me0, *vir, *stress, *sum_jij_sj, *local_volume;
     int *next, *prev, *cell_old, *cell_new;
     /* cells */
@@ -75,6 +76,20 @@ static inline double v_dot(vec a, vec b) { return a.x * b.x + a.y * b.y + a.z * 
 static inline double v_sq(vec a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 static inline double v_length(vec a) { return sqrt(v_sq(a)); }
 static inline vec v_zero(void) { vec c = {0e0, 0e0, 0e0}; return c; }
+
+/* the start of an atom's spin field sum: Hext with extfield, else zero
+ * (calculate_spin_CPU.cpp:35-39, calculate_temperature_CPU.cpp:71-75,
+ * check_temperature_CPU.cpp:59-63) */
+static inline vec hext_of(const orc_ctx* o, int a) {
+    return o->prm.extfield ? o->hext[a] : v_zero();
+}
+
+/* the force of the momentum update: with extforce, f += fext in place first
+ * (core_dp_CPU.cpp:69-71,130-132) */
+static inline vec dp_force(orc_ctx* o, int a) {
+    if (o->prm.extforce) o->f[a] = v_add(o->f[a], o->fext[a]);
+    return o->f[a];
+}
 
 /* ------------------------------------------------------------------------ */
 /* DD05 potential (DD05.cpp:6-164) and Fe exchange/Landau (JijFe.cpp:5-43)   */
@@ -645,6 +660,7 @@ static void inner_loop(orc_ctx* o, int a) {
     }
     double ri = o->rho[a];
     o->pe[a] += TRHO(o, T_F, ri);
+    if (o->prm.extfield) o->me[a] -= v_dot(o->s[a], o->hext[a]);  /* calculate_force_energy_CPU.cpp:199-201 */
     o->me[a] += TRHO(o, T_LA, ri) * si_sq + TRHO(o, T_LB, ri) * pow(si_sq, 2) +
                 TRHO(o, T_LC, ri) * pow(si_sq, 3) + TRHO(o, T_LD, ri) * pow(si_sq, 4);
 }
@@ -754,7 +770,7 @@ static inline vec landau_field(orc_ctx* o, int a, vec s, double A, double B, dou
  * update; overwrites the heff_h, sum_jij_sj and heff_hc scratch of the atoms. */
 void orc_fields(orc_ctx* o, double* h) {
     for (int a = 0; a < o->natom; ++a) {
-        o->heff_h[a] = v_zero();
+        o->heff_h[a] = hext_of(o, a);
         inner_spin(o, a);
         o->sum_jij_sj[a] = 0e0;
         inner_sum_Jij_sj(o, a);
@@ -768,7 +784,7 @@ void orc_fields(orc_ctx* o, double* h) {
 
 /* calculate_spin_CPU.cpp:33-199 (SLDHL) */
 static void calculate_spin(orc_ctx* o, int a, double dt) {
-    o->heff_h[a] = v_zero();
+    o->heff_h[a] = hext_of(o, a);
     o->heff_l[a] = v_zero();
     inner_spin(o, a);
     if (!o->prm.spinlang) {
@@ -866,7 +882,7 @@ static void core_dp_A(orc_ctx* o, double dt) {
                     noise3(o, a, nz);
                     vec ff = {nz[0], nz[1], nz[2]};
                     ff = v_times(len, ff);
-                    o->p[a] = v_add(o->p[a], v_times(dt, v_add(o->f[a], ff)));
+                    o->p[a] = v_add(o->p[a], v_times(dt, v_add(dp_force(o, a), ff)));
                     o->ave_fluct[c] = v_add(o->ave_fluct[c], ff);
                 }
                 if (active) o->ave_fluct[c] = v_divide(o->ave_fluct[c], o->count[c]);
@@ -881,7 +897,7 @@ static void core_dp_A(orc_ctx* o, double dt) {
             noise3(o, i, nz);
             vec ff = {nz[0], nz[1], nz[2]};
             ff = v_times(len, ff);
-            o->p[i] = v_add(o->p[i], v_times(dt, v_add(o->f[i], ff)));
+            o->p[i] = v_add(o->p[i], v_times(dt, v_add(dp_force(o, i), ff)));
         }
     }
     o->noise_call++;
@@ -927,7 +943,7 @@ void orc_core_dp(orc_ctx* o, double dt) {
         core_dp_B(o, dt / 2e0);
         if (o->prm.localcolmot) core_dp_C(o, dt / 2e0, 1);
     } else {
-        for (int i = 0; i < o->natom; ++i) o->p[i] = v_add(o->p[i], v_times(dt, o->f[i]));
+        for (int i = 0; i < o->natom; ++i) o->p[i] = v_add(o->p[i], v_times(dt, dp_force(o, i)));
     }
     rescale_momentum(o);
     for (int i = 0; i < o->natom; ++i) o->ke[i] = v_sq(o->p[i]) / 2e0 / o->prm.atmass;
@@ -957,7 +973,7 @@ void orc_calculate_temperature(orc_ctx* o) {
         for (int a = o->head[c]; a != -1; a = o->next[a]) {
             sum_ke += o->ke[a];
             if (o->prm.localcolmot) o->ave_p[c] = v_add(o->ave_p[c], o->p[a]);
-            o->heff_h[a] = v_zero();
+            o->heff_h[a] = hext_of(o, a);
             inner_spin(o, a);
             o->sum_R_up[c] += v_sq(v_cross(o->s[a], o->heff_h[a]));
             o->sum_R_dn[c] += v_dot(o->s[a], o->heff_h[a]);
@@ -1114,7 +1130,7 @@ void orc_check(orc_ctx* o, int step, sld_obs* out) {
         for (int c = 0; c < o->ncells; ++c) { lup += o->sum_L_up[c]; ldn += o->sum_L_dn[c]; }
     } else {
         for (int i = 0; i < N; ++i) {
-            o->heff_h[i] = v_zero();
+            o->heff_h[i] = hext_of(o, i);
             inner_spin(o, i);
             rup += v_sq(v_cross(o->s[i], o->heff_h[i]));
             rdn += v_dot(o->s[i], o->heff_h[i]);
@@ -1263,6 +1279,7 @@ orc_ctx* orc_create(const sld_params* p, const sld_atoms* in, int compute_initia
 
     o->r = ALLOCV(N); o->p = ALLOCV(N); o->f = ALLOCV(N); o->s = ALLOCV(N);
     o->heff_h = ALLOCV(N); o->heff_l = ALLOCV(N); o->heff_hc = ALLOCV(N);
+    o->hext = ALLOCV(N); o->fext = ALLOCV(N);
     o->s0 = ALLOCD(N); o->rho = ALLOCD(N); o->pe = ALLOCD(N); o->ke = ALLOCD(N);
     o->me = ALLOCD(N); o->me0 = ALLOCD(N); o->vir = ALLOCD(N); o->stress = ALLOCD(6 * N);
     o->sum_jij_sj = ALLOCD(N); o->local_volume = ALLOCD(N);
@@ -1307,9 +1324,18 @@ void orc_destroy(orc_ctx* o) {
                     o->ke, o->me, o->me0, o->vir, o->stress, o->sum_jij_sj, o->local_volume, o->next,
                     o->prev, o->cell_old, o->cell_new, o->head, o->tail, o->count, o->neigh, o->type,
                     o->group_size, o->group_members, o->Te, o->Ee, o->Tl, o->Gel, o->Ges, o->Ts_R,
-                    o->Ts_L, o->sum_R_up, o->sum_R_dn, o->sum_L_up, o->sum_L_dn, o->ave_p, o->ave_fluct};
+                    o->Ts_L, o->sum_R_up, o->sum_R_dn, o->sum_L_up, o->sum_L_dn, o->ave_p, o->ave_fluct,
+                    o->hext, o->fext};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
     free(o);
+}
+
+/* external field (eV) and force (eV/A) per atom, [natom][3]; NULL: unchanged */
+void orc_set_external(orc_ctx* o, const double* hext, const double* fext) {
+    for (int i = 0; i < o->natom; ++i) {
+        if (hext) { o->hext[i].x = hext[3 * i]; o->hext[i].y = hext[3 * i + 1]; o->hext[i].z = hext[3 * i + 2]; }
+        if (fext) { o->fext[i].x = fext[3 * i]; o->fext[i].y = fext[3 * i + 1]; o->fext[i].z = fext[3 * i + 2]; }
+    }
 }
 
 void orc_download(const orc_ctx* o, sld_atoms* out) {

@@ -44,6 +44,7 @@ void orc_check(orc_ctx* o, int step, sld_obs* out);
 void orc_core(orc_ctx* o, int step);
 int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs);
 void orc_scale_step(orc_ctx* o);
+void orc_set_external(orc_ctx* o, const double* hext, const double* fext);
 double orc_get_step(const orc_ctx* o);
 
 void orc_download(const orc_ctx* o, sld_atoms* out);

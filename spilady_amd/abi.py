@@ -39,7 +39,8 @@ class SldParams(C.Structure):
         ("reserved0", C.c_int32),
         ("displace_limit", C.c_double),
         ("phi_limit", C.c_double),
-        ("reserved", C.c_int32 * 2),
+        ("extfield", C.c_int32),
+        ("extforce", C.c_int32),
     ]
 
 
@@ -183,6 +184,8 @@ def params_from_variables(v: Dict[str, object], box, natom: int, replica: int = 
     p.changestep = int(v.get("changestep", 0))
     p.displace_limit = float(v.get("displace_limit", 0.01))
     p.phi_limit = float(v.get("phi_limit", 2.0 * np.pi / 10.0))
+    p.extfield = int(v.get("extfield", 0))
+    p.extforce = int(v.get("extforce", 0))
     return p
 
 

@@ -752,7 +752,8 @@ bool same_shared(const sld_params& a, const sld_params& b) {
            a.min_length_link_cell == b.min_length_link_cell && a.start_time == b.start_time &&
            a.interval_of_print_out == b.interval_of_print_out && a.spinlang == b.spinlang &&
            a.lattlang == b.lattlang && a.eltemp == b.eltemp && a.localcolmot == b.localcolmot &&
-           a.renormalize_energy == b.renormalize_energy && a.changestep == b.changestep;
+           a.renormalize_energy == b.renormalize_energy && a.changestep == b.changestep &&
+           a.extfield == b.extfield && a.extforce == b.extforce;
 }
 
 int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms* init, int compute_forces) {
@@ -852,6 +853,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     ph.lattlang = P0.lattlang;
     ph.eltemp = P0.eltemp;
     ph.localcolmot = P0.localcolmot;
+    ph.extfield = P0.extfield;
+    ph.extforce = P0.extforce;
 
     // tables (shared)
     std::shared_ptr<const HostTables> ht = shared_tables(P0.rcut_max);
@@ -889,6 +892,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
     TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
     TRY(dalloc(c, &D.zn, N));
+    if (P0.extfield) TRY(dalloc(c, &D.hext, N));  // zero: the reference's default field
+    if (P0.extforce) TRY(dalloc(c, &D.fext, N));
     TRY(dalloc(c, &D.pass_base, 1));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
@@ -1277,6 +1282,15 @@ int sld_scale_step(sld_ctx* c) {
 }
 
 double sld_get_step(const sld_ctx* c) { return c ? c->prm.step : 0.0; }
+
+int sld_set_external(sld_ctx* c, const double* hext, const double* fext) {
+    if (!c) return SLD_ERR_ARG;
+    if (hext && !c->phys.extfield) return set_err(c, SLD_ERR_STATE, "extfield is off in sld_params");
+    if (fext && !c->phys.extforce) return set_err(c, SLD_ERR_STATE, "extforce is off in sld_params");
+    TRY(up4(c, c->D.hext, hext, c->sel, false));
+    TRY(up4(c, c->D.fext, fext, c->sel, false));
+    return SLD_OK;
+}
 
 // ---------------------------------------------------------------------------
 // launch shapes and the autotuner

@@ -34,6 +34,7 @@ struct Dev {
     const int* crep;           // replica of each cell, -1 for padding [C]
     double4 *r, *p, *f, *s, *hh, *hl;
     double4* zn;               // Langevin normals of the current spin pass [N]
+    double4 *hext, *fext;      // external field (eV) and force (eV/A) [N]; null when off
     uint64_t* pass_base;       // noise passes consumed before the current step [1]
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
     double* stress;            // [6][N]: 11 22 33 12 23 31
@@ -55,6 +56,7 @@ struct Phys {
     double atmass, gamma_L, gamma_S;
     double rcut_pot, rcut_mag, rcut_pot_sq, rcut_mag_sq, rcut_max_sq;
     int spinlang, lattlang, eltemp, localcolmot;
+    int extfield, extforce;
 };
 
 // per-replica block of reduced scalars

@@ -372,6 +372,7 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
     double s4 = si_sq * si_sq;
     pe += F;
+    if (P.extfield) me -= dot3(si, d3(D.hext[i]));  // calculate_force_energy_CPU.cpp:199-201
     me += A * si_sq + B * s4 + C * s4 * si_sq + Dd * s4 * s4;
     D.f[i] = make_double4(f.x, f.y, f.z, 0.0);
     D.pe[i] = pe;
@@ -426,13 +427,18 @@ __device__ __forceinline__ double3 landau_field(double3 s, double A, double B, d
     return make_double3((k + hc) * s.x, (k + hc) * s.y, (k + hc) * s.z);
 }
 
-// Exchange field H = sum J s_j and S = sum J |s_j| over the rcut_mag list,
+// Exchange field H = Hext + sum J s_j (Hext with extfield, else 0:
+// calculate_spin_CPU.cpp:35-39) and S = sum J |s_j| over the rcut_mag list,
 // gathered eight neighbours at a time with all loads of a batch in flight.
-__device__ __forceinline__ void exchange_field(int a, const Dev& D, double3& H, double& S) {
+__device__ __forceinline__ void exchange_field(int a, const Dev& D, const Phys& P, double3& H, double& S) {
     const int nm = D.nbmn[a];
     const int* ids = D.nbm + (size_t)a * kMaxMag;
     const double* js = D.nbJ + (size_t)a * kMaxMag;
     double hx = 0.0, hy = 0.0, hz = 0.0, ss = 0.0;
+    if (P.extfield) {
+        const double4 he = D.hext[a];
+        hx = he.x; hy = he.y; hz = he.z;
+    }
 #pragma unroll 1
     for (int b0 = 0; b0 < nm; b0 += 8) {
         const int4 i0 = *reinterpret_cast<const int4*>(ids + b0);
@@ -591,7 +597,9 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     load_js(D, a_c, J_c);
 #pragma unroll
     for (int k = 0; k < kPipe; ++k) sj[k] = D.s[k < nm_c ? id_c[k] : a_c];
+    const bool ext = P.extfield != 0;
     double4 own_c = D.s[a_c];
+    double4 he_c = ext ? D.hext[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
     double4 z_c = lang ? D.zn[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
     LanRows lan_c = lang ? landau_rows(T, D.rho[a_c]) : LanRows{};
     int a_n = a_c, nm_n = nm_c;
@@ -606,9 +614,9 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     double4 prev_s = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int q = 0; q < n; ++q) {
         const int a = a_c;
-        // field of atom q from the gathered spins, with the previous atom's
-        // new spin patched in
-        double hx = 0.0, hy = 0.0, hz = 0.0, S = 0.0;
+        // field of atom q (from Hext, then the gathered spins) with the
+        // previous atom's new spin patched in
+        double hx = he_c.x, hy = he_c.y, hz = he_c.z, S = 0.0;
 #pragma unroll
         for (int k = 0; k < kPipe; ++k) {
             double4 v = id_c[k] == prev_a ? prev_s : sj[k];
@@ -636,6 +644,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             }
             load_js(D, a_n, J_c);
             own_c = D.s[a_n];
+            if (ext) he_c = D.hext[a_n];
             if (lang) {
                 z_c = D.zn[a_n];
                 lan_c = landau_rows(T, rho_n);
@@ -719,6 +728,18 @@ __device__ __forceinline__ double3 dp_colmot(double3 p, int l, int n, double fac
     return p;
 }
 
+// the force of the momentum update: with extforce the reference adds fext to
+// the stored force in place first (core_dp_CPU.cpp:69-71,130-132), so a
+// second core_dp before the next force evaluation adds it again
+__device__ __forceinline__ double3 dp_force(const Phys& P, const Dev& D, int a) {
+    double3 f = d3(D.f[a]);
+    if (P.extforce) {
+        f = add3(f, d3(D.fext[a]));
+        D.f[a] = make_double4(f.x, f.y, f.z, 0.0);
+    }
+    return f;
+}
+
 // One half-wave (32 lanes) per link cell, one atom per lane.
 __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int pass_off,
                                                   double* partials) {
@@ -744,7 +765,7 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int 
             if (has) {
                 const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
                 ff = mul3(len, normal3(R.seed, R.replica, a - R.atom0, pass));
-                p = add3(p, mul3(dt, add3(d3(D.f[a]), ff)));
+                p = add3(p, mul3(dt, add3(dp_force(P, D, a), ff)));
             }
             if (P.localcolmot && n > 0) {
                 double3 fs = seg_sum32(ff);
@@ -754,7 +775,7 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int 
             p = mul3(eb, p);
             if (P.localcolmot && n > 0) p = dp_colmot(p, l, n, gm * h / n, true);
         } else if (has) {
-            p = add3(p, mul3(dt, d3(D.f[a])));
+            p = add3(p, mul3(dt, dp_force(P, D, a)));
         }
         if (has) D.p[a] = make_double4(p.x, p.y, p.z, 0.0);
     }
@@ -829,7 +850,7 @@ __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
         pa = d3(D.p[a]);
         double3 H;
         double S;
-        exchange_field(a, D, H, S);
+        exchange_field(a, D, P, H, S);
         double3 s = d3(D.s[a]);
         rup = sq3(cross3(s, H));
         rdn = dot3(s, H);
@@ -1117,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_obs_atoms(Tables T, Phys P, Dev D, doub
             // check_temperature_CPU.cpp:64-77,97-128 (no eltemp)
             double3 H;
             double S;
-            exchange_field(a, D, H, S);
+            exchange_field(a, D, P, H, S);
             acc[7] = sq3(cross3(s, H));
             acc[8] = dot3(s, H);
             int n0;

@@ -32,6 +32,8 @@ constexpr double kBoltz = 8.617343e-5;
 constexpr double kSpeedOfLight = 299792458e0;
 constexpr double kUtoEv = 931.494028e6 / kSpeedOfLight / kSpeedOfLight;
 constexpr double kLengthScale = 1e-10;
+constexpr double kElG = 2.0023193043622e0;   // electron g factor (spilady.h:53-62)
+constexpr double kMuB = 5.7883817555e-5;     // eV/T
 
 [[noreturn]] void fail(const std::string& msg) {
     std::fprintf(stderr, "spilady_hip: %s\n", msg.c_str());
@@ -49,7 +51,8 @@ struct Vars {
         {"rcut_pot", "4.1"}, {"rcut_mag", "3.75"}, {"rcut_max", "4.1"}, {"min_length_link_cell", "4.11"},
         {"spinlang", "1"}, {"lattlang", "1"}, {"eltemp", "1"}, {"localcolmot", "1"},
         {"renormalizeEnergy", "1"}, {"runstep", "1"}, {"changestep", "0"},
-        {"total_production_time", "1e-12"}, {"displace_limit", "0.01"}, {"phi_limit", "0.628319"}};
+        {"total_production_time", "1e-12"}, {"displace_limit", "0.01"}, {"phi_limit", "0.628319"},
+        {"extfield", "0"}, {"extforce", "0"}};
     void read(const char* path) {
         std::ifstream in(path);
         if (!in) fail(std::string("cannot open ") + path);
@@ -238,6 +241,8 @@ int main(int argc, char** argv) {
     prm.localcolmot = prm.eltemp ? (int)v.i("localcolmot") : 0;
     prm.renormalize_energy = prm.eltemp ? (int)v.i("renormalizeEnergy") : 0;
     prm.changestep = (int)v.i("changestep");
+    prm.extfield = (int)v.i("extfield");
+    prm.extforce = (int)v.i("extforce");
     prm.displace_limit = v.d("displace_limit");
     prm.phi_limit = v.d("phi_limit");
     const bool runstep = v.i("runstep") != 0;
@@ -251,6 +256,33 @@ int main(int argc, char** argv) {
     sld_ctx* ctx = nullptr;
     sld_atoms init = a.view();
     if (sld_create(&prm, &init, 0, &ctx)) fail(ctx ? sld_last_error(ctx) : "sld_create failed");
+
+    // external_field(-1) / external_force(-1) (initial.cpp:58-63): per-atom
+    // values from extfield.in (tesla) and extforce.in (eV/A) when present;
+    // without a file the field or force stays zero
+    if (prm.extfield || prm.extforce) {
+        std::vector<double> hext, fext;
+        auto read_ext = [&](const char* name, double scale, std::vector<double>& out) {
+            std::ifstream in(name);
+            if (!in) return false;
+            int n = 0;
+            in >> n;
+            if (n != a.n) fail(std::string(name) + ": atom count differs from the configuration");
+            out.assign((size_t)3 * n, 0.0);
+            for (int i = 0; i < n; ++i) {
+                int idx;
+                double x, y, z;
+                if (!(in >> idx >> x >> y >> z)) fail(std::string("cannot read ") + name);
+                out[3 * i] = scale * x; out[3 * i + 1] = scale * y; out[3 * i + 2] = scale * z;
+            }
+            return true;
+        };
+        const double tesla_to_ev = -kElG * kMuB;  // external_field_CPU.cpp:50
+        const bool hf = prm.extfield && read_ext("extfield.in", tesla_to_ev, hext);
+        const bool ff = prm.extforce && read_ext("extforce.in", 1.0, fext);
+        if (sld_set_external(ctx, hf ? hext.data() : nullptr, ff ? fext.data() : nullptr)) fail(sld_last_error(ctx));
+        if (prm.eltemp && sld_calculate_temperature(ctx)) fail(sld_last_error(ctx));
+    }
 
     // check(-1) and write(-1) (spilady.cpp:38-40)
     sld_obs o;

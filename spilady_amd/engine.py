@@ -92,6 +92,7 @@ def load(path: Optional[str] = None):
     L.sld_total_time.argtypes = [V]
     L.sld_total_time.restype = C.c_double
     L.sld_scale_step.argtypes = [V]
+    L.sld_set_external.argtypes = [V, _dp, _dp]
     L.sld_get_step.argtypes = [V]
     L.sld_get_step.restype = C.c_double
     L.sld_last_step_timing.argtypes = [V, _dp]
@@ -283,6 +284,14 @@ class Engine(abi.EngineLike):
     def scale_step(self):
         """scale_step (scale_step_CPU.cpp:33-97)."""
         self._rc(self.L.sld_scale_step(self.h))
+
+    def set_external(self, hext=None, fext=None):
+        """Per-atom external field (eV) and force (eV/A) of the selected
+        replica, [natom][3] (needs extfield / extforce in the params)."""
+        h = None if hext is None else np.ascontiguousarray(hext, dtype=np.float64)
+        f = None if fext is None else np.ascontiguousarray(fext, dtype=np.float64)
+        self._rc(self.L.sld_set_external(self.h, None if h is None else h.ctypes.data_as(_dp),
+                                         None if f is None else f.ctypes.data_as(_dp)))
 
 
 class BatchEngine(Engine):

@@ -56,6 +56,8 @@ VARIABLE_DEFAULTS: Dict[str, object] = {
     "total_production_time": 1e-12,
     "displace_limit": 0.01,
     "phi_limit": 0.628319,
+    "extfield": 0,             # read extfield.in (tesla) at the start
+    "extforce": 0,             # read extforce.in (eV/A) at the start
 }
 
 

@@ -112,3 +112,39 @@ def test_driver_time_loop_with_changestep(tmp_path):
         assert int(row[0]) == o["step"]
         assert row[1] == fmt(o["total_time"])
         assert list(row[2:6]) == [fmt(o[k]) for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0")]
+
+
+def test_driver_reads_external_field_and_force(tmp_path):
+    """extfield.in (tesla) and extforce.in (eV/A) in the run directory, as
+    external_field_CPU.cpp / external_force_CPU.cpp read them: the driver's
+    logs against the in-process engine given the same values."""
+    from spilady_amd import abi
+    from spilady_amd.engine import Engine
+    tmp = str(tmp_path)
+    _setup(tmp, True, extfield=1, extforce=1)
+    st = io.read_con(os.path.join(tmp, "con-start.dat"))
+    n = st.natom
+    rng = np.random.default_rng(8)
+    b = rng.normal(0.0, 5.0, size=(n, 3))
+    f = rng.normal(0.0, 0.05, size=(n, 3))
+    for name, arr in (("extfield.in", b), ("extforce.in", f)):
+        with open(os.path.join(tmp, name), "w") as fh:
+            fh.write(f"{n}\n")
+            for i in range(n):
+                fh.write("%d %.17g %.17g %.17g\n" % (i, arr[i, 0], arr[i, 1], arr[i, 2]))
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    p = abi.params_from_variables(io.read_variables(os.path.join(tmp, "variables.in")), st.box, n)
+    assert p.extfield == 1 and p.extforce == 1
+    e = Engine(p, st, compute_initial_forces=False)
+    e.set_external((-2.0023193043622e0 * 5.7883817555e-5) * b, f)
+    e.calculate_temperature()
+    obs = [e.check(-1)] + e.step(0, 7)
+    e.close()
+    fmt = lambda x: float("%.15e" % x)  # noqa: E731
+    enr = io.read_log(os.path.join(tmp, "enr-drv.dat"))
+    tmpl = io.read_log(os.path.join(tmp, "tmp-drv.dat"))
+    assert enr.shape[0] == len(obs)
+    for row_e, row_t, o in zip(enr, tmpl, obs):
+        assert list(row_e[2:]) == [fmt(o[k]) for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0", "ave_ee")]
+        assert list(row_t[2:]) == [fmt(o[k]) for k in ("tl", "ts_r", "ts_l", "te")]

@@ -207,3 +207,30 @@ def test_changestep_matches_oracle():
     assert np.sum(steps[1:] < steps[:-1]) >= 1 and np.sum(steps[1:] > steps[:-1]) >= 1
     _compare(eng, orc, "changestep", cells=True)
     eng.close(); orc.close()
+
+
+def test_external_field_and_force_match_oracle():
+    """extfield and extforce: random per-atom fields (about 10 T) and forces
+    (0.1 eV/A), 8 steps with thermostats and the electron temperature; the
+    observables and the final state at the tolerance."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(8, 600.0, 7, thermostats=True, eltemp=True, localcolmot=True, tilt=20.0)
+    p.extfield = 1
+    p.extforce = 1
+    st = consistent_state(p, at)
+    n = p.natom
+    rng = np.random.default_rng(4)
+    hext = (-2.0023193043622e0 * 5.7883817555e-5) * rng.normal(0.0, 10.0, size=(n, 3))
+    fext = rng.normal(0.0, 0.1, size=(n, 3))
+    eng = Engine(p, st, compute_initial_forces=False)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_PHILOX)
+    for x in (eng, orc):
+        x.set_external(hext, fext)
+        x.calculate_temperature()
+    oa = orc.step(0, 8)
+    ob = eng.step(0, 8)
+    for x, y in zip(oa, ob):
+        for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0", "tl", "ts_r", "ts_l", "te"):
+            assert abs(x[k] - y[k]) <= TOL * max(abs(x[k]), 1e-300), (x["step"], k, x[k], y[k])
+    _compare(eng, orc, "external", cells=True)
+    eng.close(); orc.close()

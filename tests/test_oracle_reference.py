@@ -202,3 +202,47 @@ def test_live_reference_changestep():
     assert np.sum(steps[1:] < steps[:-1]) >= 2 and np.sum(steps[1:] > steps[:-1]) >= 2
     for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.skipif(not refpy.available_ext(),
+                    reason="reference with extfield/extforce not built (oracle/Makefile.ref OUT=_ref/ext)")
+def test_live_reference_external_field_and_force():
+    """extfield and extforce: per-atom fields (read by the reference from
+    extfield.in in tesla, converted to eV as -g muB B) and forces (extforce.in,
+    eV/A); 6 steps of core() with thermostats and the electron temperature,
+    bit for bit. The in-place f += fext of core_dp is part of what is pinned."""
+    at = configs.make_bcc_fe(8, 600.0, 7, tilt_max_deg=20.0)
+    v = configs.fe_variables(600.0, 7, thermostats=True, eltemp=True, localcolmot=True, renormalize=False)
+    st = refpy.consistent_start(v, at)
+    n = st.natom
+    rng = np.random.default_rng(4)
+    b_tesla = rng.normal(0.0, 10.0, size=(n, 3))
+    fext = rng.normal(0.0, 0.1, size=(n, 3))
+    d = tempfile.mkdtemp()
+    for name, arr in (("extfield.in", b_tesla), ("extforce.in", fext)):
+        with open(os.path.join(d, name), "w") as fh:
+            fh.write(f"{n}\n")
+            for i in range(n):
+                fh.write("%d %.17g %.17g %.17g\n" % (i, arr[i, 0], arr[i, 1], arr[i, 2]))
+    ref = refpy.Reference(refpy.REF_LIB_EXT)
+    ref.init_memory(d, v, st)
+    for cur in range(6):
+        ref.run(cur, 1, False)
+    a = ref.atoms()
+    ref.free()
+    # external_field_CPU.cpp:50: Hext = (-el_g*muB) * B
+    hext = (-2.0023193043622e0 * 5.7883817555e-5) * b_tesla
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), st.box, n)
+    p.renormalize_energy = 0
+    p.extfield = 1
+    p.extforce = 1
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    o.set_external(hext, fext)
+    # initial.cpp reads the field before its calculate_temperature()
+    o.calculate_temperature()
+    for cur in range(6):
+        o.core(cur)
+    b = o.download()
+    o.close()
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0", "heff_h"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
