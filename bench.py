@@ -367,7 +367,7 @@ def main() -> int:
     # core_dTe graph and as one replayed graph per step
     graph_ms = {}
     s1 = s0 + len([x for x in args.kernels.split(",") if x])
-    for mode in (0, 1, 2):
+    for mode in ((0, 1, 2) if args.graph_steps > 0 else ()):
         eng.set_graph_mode(mode)
         eng.step(s1, 1)
         t0 = time.perf_counter()

@@ -179,6 +179,27 @@ int sld_step(sld_ctx* ctx, int first_step, int nsteps, sld_obs* obs, int* nobs);
 int sld_scale_step(sld_ctx* ctx);
 /* The current time step (s). */
 double sld_get_step(const sld_ctx* ctx);
+
+/* Berendsen-type barostats, the control.h STRESS and PRESSURE options
+ * (scale_stress_CPU.cpp:33-86, scale_pressure_CPU.cpp:33-61). Off until set;
+ * single-system contexts only. The box changes, the link-cell counts do not
+ * (as in the reference). */
+typedef struct sld_barostat {
+    int32_t stress_on;                  /* STRESS: box matrix toward the target stresses */
+    int32_t pressure_on;                /* PRESSURE: isotropic, toward the target pressure */
+    int32_t interval_of_scale_stress;   /* scale after every this many steps */
+    int32_t interval_of_scale_pressure;
+    double stress[6];                   /* GPa: stress_xx yx yy zx zy zz (read_variables.cpp:197-203) */
+    double pressure;                    /* GPa (read_variables.cpp:209) */
+    double baro_damping_time;           /* s (read_variables.cpp:213) */
+} sld_barostat;
+int sld_set_barostat(sld_ctx* ctx, const sld_barostat* b);
+/* scale(step) (scale.cpp:31-44): STRESS, then PRESSURE at their intervals,
+ * then changestep. It uses the stresses and pressure of the last check().
+ * sld_step calls it after every step itself. */
+int sld_scale(sld_ctx* ctx, int step);
+/* The current box matrix of the selected replica (d.xx d.yx d.yy d.zx d.zy d.zz). */
+int sld_get_box(sld_ctx* ctx, double* box);
 /* The per-atom external field Hext (eV, i.e. -g muB B; external_field_CPU.cpp:
  * 38-53) and external force fext (eV/A; external_force_CPU.cpp:38-50) of the
  * selected replica, [natom][3] each in atom order; NULL leaves one unchanged.

@@ -78,6 +78,9 @@ def lib():
         L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normal3.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint64, _dp]
         L.orc_scale_step.argtypes = [C.c_void_p]
+        L.orc_set_barostat.argtypes = [C.c_void_p, C.POINTER(abi.SldBarostat)]
+        L.orc_scale.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_box.argtypes = [C.c_void_p, _dp]
         L.orc_set_external.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_get_step.argtypes = [C.c_void_p]
         L.orc_get_step.restype = C.c_double
@@ -196,6 +199,19 @@ class Oracle(abi.EngineLike):
 
     def scale_step(self):
         self.L.orc_scale_step(self.h)
+
+    def set_barostat(self, b: abi.SldBarostat):
+        self._baro = b
+        self.L.orc_set_barostat(self.h, C.byref(b))
+
+    def scale(self, step: int):
+        """scale(step): barostats at their intervals, then changestep."""
+        self.L.orc_scale(self.h, int(step))
+
+    def box(self) -> np.ndarray:
+        b = np.zeros(6)
+        self.L.orc_get_box(self.h, b.ctypes.data_as(_dp))
+        return b
 
     def set_external(self, hext=None, fext=None):
         """Per-atom external field (eV) and force (eV/A), [natom][3]."""

@@ -128,6 +128,13 @@ int ref_init_memory(const char* dir, int n, const double* dbox, const double* r,
     allocate_cells();
     calculate_temperature();
     total_time = start_time;
+    // initial.cpp:93-98: the barostats' clocks start at the start time
+#ifdef PRESSURE
+    last_total_time_pressure = total_time;
+#endif
+#ifdef STRESS
+    last_total_time_stress = total_time;
+#endif
     initialize_random_number();
     check(-1);
     g_live = true;
@@ -157,8 +164,10 @@ void ref_core_dr(double dt) { Quiet q(false); core_dr(dt); }
 void ref_core_dTe(double dt) { Quiet q(false); core_dTe(dt); }
 void ref_calculate_temperature(void) { Quiet q(false); calculate_temperature(); }
 void ref_check(int cur) { Quiet q(true); check(cur); }
-// scale() (scale.cpp:31-44): with the shipped control.h, scale_step only
+// scale() (scale.cpp:31-44): with the shipped control.h, scale_step only;
+// a build with -DSTRESS -DPRESSURE adds the two barostats before it
 void ref_scale(int cur) { Quiet q(true); scale(cur); }
+void ref_get_box(double* b) { b[0] = d.xx; b[1] = d.yx; b[2] = d.yy; b[3] = d.zx; b[4] = d.zy; b[5] = d.zz; }
 
 int ref_natom(void) { return natom; }
 int ref_ncells(void) { return ncells; }

@@ -27,7 +27,10 @@ REF_KEYS = ["OMP_threads", "out_body", "step", "interval_of_print_out",
             "interval_of_config_out", "in_config", "kappa_e", "start_time",
             "atmass", "temperature", "gamma_L_over_mass", "gamma_S_HL",
             "random_seed", "rcut_pot", "rcut_mag", "rcut_max",
-            "min_length_link_cell", "displace_limit", "phi_limit"]
+            "min_length_link_cell", "displace_limit", "phi_limit",
+            "interval_of_scale_stress", "stress_xx", "stress_yx", "stress_yy",
+            "stress_zx", "stress_zy", "stress_zz", "interval_of_scale_pressure",
+            "pressure", "baro_damping_time"]
 
 # Fixed switches of the reference build (shipped control.h).
 REF_SWITCHES = {"spinlang": 1, "lattlang": 1, "eltemp": 1, "localcolmot": 1,
@@ -44,6 +47,14 @@ def available() -> bool:
 
 def available_ext() -> bool:
     return os.path.exists(REF_LIB_EXT)
+
+
+# the same sources built with -DSTRESS -DPRESSURE (Makefile.ref OUT=_ref/baro)
+REF_LIB_BARO = os.path.join(HERE, "_ref", "baro", "libspilady_ref.so")
+
+
+def available_baro() -> bool:
+    return os.path.exists(REF_LIB_BARO)
 
 
 def _ptr(a: np.ndarray):
@@ -64,6 +75,7 @@ class Reference:
             getattr(L, nm).argtypes = [C.c_double]
         L.ref_check.argtypes = [C.c_int]
         L.ref_scale.argtypes = [C.c_int]
+        L.ref_get_box.argtypes = [_dp]
         for nm in ("ref_step", "ref_total_time", "ref_virial", "ref_initial_ave_energy"):
             getattr(L, nm).restype = C.c_double
         L.ref_scalar.restype = C.c_double
@@ -168,6 +180,11 @@ class Reference:
         self.lib.ref_get_groups(sizes.ctypes.data_as(_ip), members.ctypes.data_as(_ip))
         g = self.lib.ref_ngroups()
         return sizes[:g].copy(), members
+
+    def box(self) -> np.ndarray:
+        b = np.zeros(6)
+        self.lib.ref_get_box(_ptr(b))
+        return b
 
     def scalar(self, name: str) -> float:
         return self.lib.ref_scalar(name.encode())

@@ -60,6 +60,10 @@ struct orc_ctx {
     int spare;
     double rand1, rand2;
     uint64_t noise_call;       /* counter-based noise: one id per noise pass */
+    /* barostats (STRESS, PRESSURE) and the last check()'s values they act on */
+    sld_barostat baro;
+    double last_t_stress, last_t_pressure;
+    double last_stress[6], last_pressure;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1178,6 +1182,8 @@ void orc_check(orc_ctx* o, int step, sld_obs* out) {
     double tmp = 2e0 / 3e0 * sk / N;
     ob.pressure = density * (tmp - o->virial / (3e0 * N));
     ob.pressure *= 160.217653e0;
+    memcpy(o->last_stress, ob.ave_stress, sizeof(o->last_stress));
+    o->last_pressure = ob.pressure;
     if (out) *out = ob;
 }
 
@@ -1224,6 +1230,71 @@ void orc_scale_step(orc_ctx* o) {
 
 double orc_get_step(const orc_ctx* o) { return o->prm.step; }
 
+/* box_length, volume and inverse of a new box (scale_pressure_CPU.cpp:47-55) */
+static void new_box(orc_ctx* o, const double* d) {
+    memcpy(o->d, d, sizeof(o->d));
+    inverse_box(o->d, o->inv);
+    o->box_length.x = fabs(o->d[0]);
+    o->box_length.y = sqrt(o->d[1] * o->d[1] + o->d[2] * o->d[2]);
+    o->box_length.z = sqrt(o->d[3] * o->d[3] + o->d[4] * o->d[4] + o->d[5] * o->d[5]);
+    o->box_volume = o->d[0] * o->d[2] * o->d[5];
+}
+
+/* scale_stress_CPU.cpp:33-86: each box entry moves with the difference
+ * between the last checked average stress and the target; the atoms keep
+ * their coordinates in the old box's frame. ave_stress is 11 22 33 12 23 31,
+ * the targets are xx yx yy zx zy zz. */
+static void scale_stress(orc_ctx* o) {
+    double delta_time = o->total_time - o->last_t_stress;
+    o->last_t_stress = o->total_time;
+    double pre = delta_time / o->baro.baro_damping_time / 100e0;
+    const double *a = o->last_stress, *t = o->baro.stress;
+    double fxx = pre * (a[0] - t[0]), fyx = pre * (a[3] - t[1]), fyy = pre * (a[1] - t[2]);
+    double fzx = pre * (a[5] - t[3]), fzy = pre * (a[4] - t[4]), fzz = pre * (a[2] - t[5]);
+    double d[6], I[6];
+    memcpy(d, o->d, sizeof(d));
+    memcpy(I, o->inv, sizeof(I));
+    d[0] = (1e0 + fxx) * d[0];
+    d[1] = (1e0 + fxx) * d[1] + fyx * (d[2] + d[0]) / 2e0;
+    d[2] = (1e0 + fyy) * d[2];
+    d[3] = (1e0 + fxx) * d[3] + fzx * (d[5] + d[0]) / 2e0;
+    d[4] = (1e0 + fyy) * d[4] + fzy * (d[5] + d[2]) / 2e0;
+    d[5] = (1e0 + fzz) * d[5];
+    for (int i = 0; i < o->natom; ++i) {
+        vec r = o->r[i], q;
+        q.x = I[0] * r.x + I[1] * r.y + I[3] * r.z;
+        q.y = I[2] * r.y + I[4] * r.z;
+        q.z = I[5] * r.z;
+        o->r[i].x = d[0] * q.x + d[1] * q.y + d[3] * q.z;
+        o->r[i].y = d[2] * q.y + d[4] * q.z;
+        o->r[i].z = d[5] * q.z;
+    }
+    new_box(o, d);
+}
+
+/* scale_pressure_CPU.cpp:33-61: isotropic factor from the last checked pressure */
+static void scale_pressure(orc_ctx* o) {
+    double delta_time = o->total_time - o->last_t_pressure;
+    o->last_t_pressure = o->total_time;
+    double factor = pow(1e0 + delta_time / o->baro.baro_damping_time * (o->last_pressure - o->baro.pressure) / 100e0,
+                        1e0 / 3e0);
+    double d[6];
+    for (int k = 0; k < 6; ++k) d[k] = o->d[k] * factor;
+    new_box(o, d);
+    for (int i = 0; i < o->natom; ++i) o->r[i] = v_times(factor, o->r[i]);
+}
+
+void orc_set_barostat(orc_ctx* o, const sld_barostat* b) { o->baro = *b; }
+
+/* scale.cpp:31-44 */
+void orc_scale(orc_ctx* o, int step) {
+    if (o->baro.stress_on && (step + 1) % o->baro.interval_of_scale_stress == 0) scale_stress(o);
+    if (o->baro.pressure_on && (step + 1) % o->baro.interval_of_scale_pressure == 0) scale_pressure(o);
+    if (o->prm.changestep) orc_scale_step(o);
+}
+
+void orc_get_box(const orc_ctx* o, double* box) { memcpy(box, o->d, sizeof(o->d)); }
+
 /* spilady.cpp:43-57: core, time, check on printed steps, scale */
 int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs) {
     int k = 0;
@@ -1235,7 +1306,7 @@ int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs) {
             orc_check(o, s, obs ? obs + k : NULL);
             ++k;
         }
-        if (o->prm.changestep) orc_scale_step(o);
+        orc_scale(o, s);
     }
     return k;
 }
@@ -1308,6 +1379,7 @@ orc_ctx* orc_create(const sld_params* p, const sld_atoms* in, int compute_initia
     }
     if (o->prm.eltemp) orc_calculate_temperature(o);
     o->total_time = p->start_time;
+    o->last_t_stress = o->last_t_pressure = o->total_time;  /* initial.cpp:93-98 */
     /* random_number_CPU.cpp:46-48 for thread 0 */
     o->m_w = (uint32_t)(0 + (int)p->random_seed);
     o->m_z = (uint32_t)(0 * 3 + (int)p->random_seed * 2);

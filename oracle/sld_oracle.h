@@ -46,6 +46,11 @@ int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs);
 void orc_scale_step(orc_ctx* o);
 void orc_set_external(orc_ctx* o, const double* hext, const double* fext);
 double orc_get_step(const orc_ctx* o);
+/* the Berendsen barostats (scale_stress_CPU.cpp, scale_pressure_CPU.cpp) and
+ * scale() (scale.cpp:31-44): stress, then pressure, then changestep */
+void orc_set_barostat(orc_ctx* o, const sld_barostat* b);
+void orc_scale(orc_ctx* o, int step);
+void orc_get_box(const orc_ctx* o, double* box);
 
 void orc_download(const orc_ctx* o, sld_atoms* out);
 void orc_upload(orc_ctx* o, const sld_atoms* in);

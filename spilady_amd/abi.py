@@ -76,6 +76,30 @@ SHAPE_CANDIDATES = {"spin_block": (32, 64, 128), "sweep_block": (64, 128, 256),
                     "temperature_block": (64, 128, 256)}
 
 
+class SldBarostat(C.Structure):
+    _fields_ = [("stress_on", C.c_int32), ("pressure_on", C.c_int32),
+                ("interval_of_scale_stress", C.c_int32), ("interval_of_scale_pressure", C.c_int32),
+                ("stress", C.c_double * 6), ("pressure", C.c_double), ("baro_damping_time", C.c_double)]
+
+
+STRESS_KEYS = ("stress_xx", "stress_yx", "stress_yy", "stress_zx", "stress_zy", "stress_zz")
+
+
+def barostat_from_variables(v: Dict[str, object]) -> SldBarostat:
+    """The STRESS / PRESSURE switches and their variables.in values
+    (read_variables.cpp:196-214; GPa and seconds, no conversion)."""
+    b = SldBarostat()
+    b.stress_on = int(v.get("STRESS", 0))
+    b.pressure_on = int(v.get("PRESSURE", 0))
+    b.interval_of_scale_stress = int(v.get("interval_of_scale_stress", 1))
+    b.interval_of_scale_pressure = int(v.get("interval_of_scale_pressure", 1))
+    for k, key in enumerate(STRESS_KEYS):
+        b.stress[k] = float(v.get(key, 0.0))
+    b.pressure = float(v.get("pressure", 0.0))
+    b.baro_damping_time = float(v.get("baro_damping_time", 1e-13))
+    return b
+
+
 VEC_FIELDS = ("r", "p", "f", "s", "heff_h", "heff_l")
 SCALAR_FIELDS = ("rho", "pe", "ke", "me", "me0", "vir", "sum_jij_sj")
 CELL_FIELDS = ("te", "ee", "tl", "ts_r", "ts_l", "gel", "ges", "sum_r_up", "sum_r_dn",

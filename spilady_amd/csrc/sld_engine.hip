@@ -85,6 +85,10 @@ struct sld_ctx {
     double total_time = 0.0;
     uint64_t pass = 0;                       // noise pass counter
     uint64_t pass_dev = 0;                   // value of the device counter D.pass_base
+    // barostats and the last check()'s stresses and pressure they act on
+    sld_barostat baro{};
+    double last_t_stress = 0.0, last_t_pressure = 0.0;
+    sld_obs last_obs{};
     bool nb_dirty = true;
     double last_ms = 0.0;
     // per-kernel event timing of one named kernel
@@ -556,6 +560,106 @@ int do_check(sld_ctx* c, int step, sld_obs* d_out) {
     return SLD_OK;
 }
 
+// the geometry a box matrix determines (vec_utils_CPU.cpp:135-148,
+// core_dTe_CPU.cpp:143-147): inverse, volume, cell edges; the link-cell
+// counts stay
+void set_box(sld_ctx* c, int r, const double* d) {
+    ReplicaHost& h = c->rh[r];
+    Rep& R = c->reps[r];
+    Box& b = R.box;
+    for (int k = 0; k < 6; ++k) {
+        b.d[k] = d[k];
+        h.prm.box[k] = d[k];
+    }
+    b.inv[0] = 1e0 / d[0];
+    b.inv[1] = -d[1] / (d[0] * d[2]);
+    b.inv[2] = 1e0 / d[2];
+    b.inv[3] = (d[1] * d[4] - d[2] * d[3]) / (d[0] * d[2] * d[5]);
+    b.inv[4] = -d[4] / (d[2] * d[5]);
+    b.inv[5] = 1e0 / d[5];
+    h.vol = d[0] * d[2] * d[5];
+    const double blx = std::fabs(d[0]);
+    const double bly = std::sqrt(d[1] * d[1] + d[2] * d[2]);
+    const double blz = std::sqrt(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+    h.dx2 = std::pow(blx / h.hc.nx, 2);
+    h.dy2 = std::pow(bly / h.hc.ny, 2);
+    h.dz2 = std::pow(blz / h.hc.nz, 2);
+    R.box_volume = h.vol;
+    if (r == 0)
+        for (int k = 0; k < 6; ++k) c->prm.box[k] = d[k];
+}
+
+int upload_reps(sld_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync((void*)c->D.rep, c->reps.data(), c->reps.size() * sizeof(Rep), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
+// scale_stress_CPU.cpp:33-86: the box matrix moves toward the target
+// stresses, the atoms keep their fractional coordinates
+int do_scale_stress(sld_ctx* c) {
+    const double delta_time = c->total_time - c->last_t_stress;
+    c->last_t_stress = c->total_time;
+    const double ftmass = 100e0;
+    const double pre_fact = delta_time / c->baro.baro_damping_time / ftmass;
+    const double* sv = c->last_obs.ave_stress;  // 11 22 33 12 23 31
+    const double* t = c->baro.stress;           // xx yx yy zx zy zz
+    const double fxx = pre_fact * (sv[0] - t[0]), fyx = pre_fact * (sv[3] - t[1]);
+    const double fyy = pre_fact * (sv[1] - t[2]), fzx = pre_fact * (sv[5] - t[3]);
+    const double fzy = pre_fact * (sv[4] - t[4]), fzz = pre_fact * (sv[2] - t[5]);
+    const Box old_box = c->reps[0].box;
+    double d[6];
+    for (int k = 0; k < 6; ++k) d[k] = old_box.d[k];
+    // in the reference's order: each line sees the entries updated above it
+    d[0] = (1e0 + fxx) * d[0];
+    d[1] = (1e0 + fxx) * d[1] + fyx * (d[2] + d[0]) / 2e0;
+    d[2] = (1e0 + fyy) * d[2];
+    d[3] = (1e0 + fxx) * d[3] + fzx * (d[5] + d[0]) / 2e0;
+    d[4] = (1e0 + fyy) * d[4] + fzy * (d[5] + d[2]) / 2e0;
+    d[5] = (1e0 + fzz) * d[5];
+    set_box(c, 0, d);
+    LAUNCH(c, k_rebox_r, nblk(c->D.N, 256), 256, c->D, old_box, c->reps[0].box);
+    TRY(upload_reps(c));
+    c->nb_dirty = true;
+    return SLD_OK;
+}
+
+// scale_pressure_CPU.cpp:33-61: isotropic scaling toward the target pressure
+int do_scale_pressure(sld_ctx* c) {
+    const double delta_time = c->total_time - c->last_t_pressure;
+    c->last_t_pressure = c->total_time;
+    const double ftmass = 100e0;
+    const double factor = std::pow(
+        1e0 + delta_time / c->baro.baro_damping_time * (c->last_obs.pressure - c->baro.pressure) / ftmass, 1e0 / 3e0);
+    double d[6];
+    for (int k = 0; k < 6; ++k) d[k] = c->reps[0].box.d[k] * factor;
+    set_box(c, 0, d);
+    LAUNCH(c, k_scale_r, nblk(c->D.N, 256), 256, c->D, factor);
+    TRY(upload_reps(c));
+    c->nb_dirty = true;
+    return SLD_OK;
+}
+
+int do_scale_step(sld_ctx* c);
+void drop_graphs(sld_ctx* c);
+
+// scale(step) (scale.cpp:31-44)
+int do_scale(sld_ctx* c, int step) {
+    bool boxed = false;
+    if (c->baro.stress_on && (step + 1) % c->baro.interval_of_scale_stress == 0) {
+        TRY(do_scale_stress(c));
+        boxed = true;
+    }
+    if (c->baro.pressure_on && (step + 1) % c->baro.interval_of_scale_pressure == 0) {
+        TRY(do_scale_pressure(c));
+        boxed = true;
+    }
+    if (boxed) drop_graphs(c);
+    if (c->prm.changestep) TRY(do_scale_step(c));
+    return SLD_OK;
+}
+
 // scale_step_CPU.cpp:33-97: shrink dt by 0.8 when the step just taken moved
 // an atom farther than displace_limit or turned a spin by more than
 // phi_limit, else grow it by 1.05
@@ -1015,7 +1119,9 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     c->pass = 0;
     c->pass_dev = 0;
     c->cl_first = c->D.cl;
+    c->last_t_stress = c->last_t_pressure = c->total_time;  // initial.cpp:93-98
     TRY(do_check(c, -1, c->d_obs));
+    HIPCHK(c, hipMemcpyAsync(&c->last_obs, c->d_obs, sizeof(sld_obs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return read_flags(c);
 }
@@ -1125,6 +1231,7 @@ int sld_calculate_temperature(sld_ctx* c) {
 int sld_check(sld_ctx* c, int step, sld_obs* out) {
     TRY(do_check(c, step, c->d_obs));
     if (out) HIPCHK(c, hipMemcpyAsync(out, c->d_obs, sizeof(sld_obs) * c->R, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->last_obs, c->d_obs, sizeof(sld_obs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return read_flags(c);
 }
@@ -1154,10 +1261,15 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
         c->total_time += c->prm.step;
         if ((s + 1) % every == 0) {
             TRY(do_check(c, s, c->d_obs + (size_t)k * c->R));
+            if (c->baro.stress_on || c->baro.pressure_on) {
+                HIPCHK(c, hipMemcpyAsync(&c->last_obs, c->d_obs + (size_t)k * c->R, sizeof(sld_obs),
+                                         hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+            }
             ++k;
         }
         // scale() (spilady.cpp:50-57, scale.cpp:31-44)
-        if (c->prm.changestep) TRY(do_scale_step(c));
+        TRY(do_scale(c, s));
     }
     LAUNCH(c, k_check_nonfinite, nblk(c->D.N, 256), 256, c->D);
     HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -1282,6 +1394,30 @@ int sld_scale_step(sld_ctx* c) {
 }
 
 double sld_get_step(const sld_ctx* c) { return c ? c->prm.step : 0.0; }
+
+int sld_set_barostat(sld_ctx* c, const sld_barostat* b) {
+    if (!c || !b) return SLD_ERR_ARG;
+    if ((b->stress_on || b->pressure_on) && c->R > 1)
+        return set_err(c, SLD_ERR_ARG, "barostats run on single-system contexts only");
+    if (b->stress_on && b->interval_of_scale_stress < 1) return set_err(c, SLD_ERR_ARG, "stress interval < 1");
+    if (b->pressure_on && b->interval_of_scale_pressure < 1) return set_err(c, SLD_ERR_ARG, "pressure interval < 1");
+    if ((b->stress_on || b->pressure_on) && !(b->baro_damping_time > 0.0))
+        return set_err(c, SLD_ERR_ARG, "baro_damping_time must be positive");
+    c->baro = *b;
+    return SLD_OK;
+}
+
+int sld_scale(sld_ctx* c, int step) {
+    if (!c) return SLD_ERR_ARG;
+    TRY(do_scale(c, step));
+    return read_flags(c);
+}
+
+int sld_get_box(sld_ctx* c, double* box) {
+    if (!c || !box) return SLD_ERR_ARG;
+    for (int k = 0; k < 6; ++k) box[k] = c->reps[c->sel].box.d[k];
+    return SLD_OK;
+}
 
 int sld_set_external(sld_ctx* c, const double* hext, const double* fext) {
     if (!c) return SLD_ERR_ARG;

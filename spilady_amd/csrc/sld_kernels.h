@@ -70,6 +70,8 @@ __global__ void k_fp(Tables T, Dev D);
 __global__ void k_force(Tables T, Phys P, Dev D);
 __global__ void k_spin_noise(Dev D, int pass_off);
 __global__ void k_step_max(Dev D, double* partials);
+__global__ void k_scale_r(Dev D, double factor);
+__global__ void k_rebox_r(Dev D, Box old_box, Box new_box);
 __global__ void k_reduce_max2(const double* partials, int nblk, double* out);
 __global__ void k_pass_advance(Dev D, int n);
 __global__ void k_spin_phase(const int* cells, int ncell, int backward, double dt, Tables T, Phys P,

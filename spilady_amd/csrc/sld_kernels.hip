@@ -1063,6 +1063,31 @@ __global__ __launch_bounds__(256) void k_dte_finish(double h, Phys P, Dev D, con
 
 
 // ---------------------------------------------------------------------------
+// barostats: positions follow the box (scale_pressure_CPU.cpp:59,
+// scale_stress_CPU.cpp:67-80)
+
+// PRESSURE: r = factor r
+__global__ __launch_bounds__(256) void k_scale_r(Dev D, double factor) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.N || D.arep[a] < 0) return;
+    const double4 r = D.r[a];
+    D.r[a] = make_double4(factor * r.x, factor * r.y, factor * r.z, r.w);
+}
+
+// STRESS: fractional coordinates of the old box, real ones of the new box
+__global__ __launch_bounds__(256) void k_rebox_r(Dev D, Box old_box, Box new_box) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.N || D.arep[a] < 0) return;
+    const double4 r = D.r[a];
+    const double* iv = old_box.inv;
+    const double* d = new_box.d;
+    const double qx = iv[0] * r.x + iv[1] * r.y + iv[3] * r.z;
+    const double qy = iv[2] * r.y + iv[4] * r.z;
+    const double qz = iv[5] * r.z;
+    D.r[a] = make_double4(d[0] * qx + d[1] * qy + d[3] * qz, d[2] * qy + d[4] * qz, d[5] * qz, r.w);
+}
+
+// ---------------------------------------------------------------------------
 // scale_step (scale_step_CPU.cpp:35-70): the largest |p|^2 and |Heff_H|^2
 
 __global__ __launch_bounds__(256) void k_step_max(Dev D, double* partials) {

@@ -52,7 +52,11 @@ struct Vars {
         {"spinlang", "1"}, {"lattlang", "1"}, {"eltemp", "1"}, {"localcolmot", "1"},
         {"renormalizeEnergy", "1"}, {"runstep", "1"}, {"changestep", "0"},
         {"total_production_time", "1e-12"}, {"displace_limit", "0.01"}, {"phi_limit", "0.628319"},
-        {"extfield", "0"}, {"extforce", "0"}};
+        {"extfield", "0"}, {"extforce", "0"},
+        // Berendsen barostats: control.h STRESS / PRESSURE, default.h:177-193
+        {"STRESS", "0"}, {"PRESSURE", "0"}, {"interval_of_scale_stress", "1"}, {"stress_xx", "0"},
+        {"stress_yx", "0"}, {"stress_yy", "0"}, {"stress_zx", "0"}, {"stress_zy", "0"}, {"stress_zz", "0"},
+        {"interval_of_scale_pressure", "1"}, {"pressure", "0"}, {"baro_damping_time", "1e-13"}};
     void read(const char* path) {
         std::ifstream in(path);
         if (!in) fail(std::string("cannot open ") + path);
@@ -122,7 +126,8 @@ std::string numbered(const char* front, const std::string& body, int n_output) {
 
 // check_energy/temperature/stress/pressure/spin (check.cpp:31-47): one line
 // per printed step in each of enr-, tmp-, str-, prs- and spn-
-void append_logs(const std::string& body, const sld_obs& o, const sld_params& prm) {
+// box: the box matrix at the check (it changes under the barostats)
+void append_logs(const std::string& body, const sld_obs& o, const sld_params& prm, const double* box) {
     const int eltemp = prm.eltemp;
     FILE* e = std::fopen(("enr-" + body + ".dat").c_str(), "a");
     FILE* t = std::fopen(("tmp-" + body + ".dat").c_str(), "a");
@@ -132,7 +137,7 @@ void append_logs(const std::string& body, const sld_obs& o, const sld_params& pr
     if (!e || !t || !s || !st || !pr) fail("cannot append to the enr-/tmp-/spn-/str-/prs- files");
     // check_stress_CPU.cpp:68-81 and check_pressure_CPU.cpp:49-55: step, time,
     // the box matrix, the density, then the stresses or the pressure
-    const double* b = prm.box;
+    const double* b = box;
     const double density = (double)prm.natom / (b[0] * b[2] * b[5]);
     const double* sv = o.ave_stress;
     std::fprintf(st, "%d %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e\n",
@@ -160,6 +165,7 @@ void append_logs(const std::string& body, const sld_obs& o, const sld_params& pr
 void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a, int eltemp) {
     sld_atoms v = a.view();
     if (sld_download(ctx, &v)) fail(sld_last_error(ctx));
+    if (sld_get_box(ctx, a.box)) fail(sld_last_error(ctx));
     const double t = sld_total_time(ctx);
     FILE* f = std::fopen(numbered("con-", body, n_output).c_str(), "w");
     if (!f) fail("cannot write the con- file");
@@ -245,6 +251,16 @@ int main(int argc, char** argv) {
     prm.extforce = (int)v.i("extforce");
     prm.displace_limit = v.d("displace_limit");
     prm.phi_limit = v.d("phi_limit");
+    sld_barostat baro = {};
+    baro.stress_on = (int)v.i("STRESS");
+    baro.pressure_on = (int)v.i("PRESSURE");
+    baro.interval_of_scale_stress = (int)v.i("interval_of_scale_stress");
+    baro.interval_of_scale_pressure = (int)v.i("interval_of_scale_pressure");
+    const char* stress_keys[6] = {"stress_xx", "stress_yx", "stress_yy", "stress_zx", "stress_zy", "stress_zz"};
+    for (int k = 0; k < 6; ++k) baro.stress[k] = v.d(stress_keys[k]);
+    baro.pressure = v.d("pressure");
+    baro.baro_damping_time = v.d("baro_damping_time");
+    const bool barostats = baro.stress_on || baro.pressure_on;
     const bool runstep = v.i("runstep") != 0;
     const double total_production_time = v.d("total_production_time");
     const std::string body = v.s("out_body");
@@ -284,10 +300,14 @@ int main(int argc, char** argv) {
         if (prm.eltemp && sld_calculate_temperature(ctx)) fail(sld_last_error(ctx));
     }
 
+    if (barostats && sld_set_barostat(ctx, &baro)) fail(sld_last_error(ctx));
+
     // check(-1) and write(-1) (spilady.cpp:38-40)
     sld_obs o;
+    double box[6];
     if (sld_check(ctx, -1, &o)) fail(sld_last_error(ctx));
-    append_logs(body, o, prm);
+    if (sld_get_box(ctx, box)) fail(sld_last_error(ctx));
+    append_logs(body, o, prm, box);
     write_config(ctx, body, 0, a, prm.eltemp);
 
     std::vector<sld_obs> obs;
@@ -299,8 +319,9 @@ int main(int argc, char** argv) {
         long long s = 0;
         while (sld_total_time(ctx) <= total_production_time + 1e-18) {
             int nobs = 0;
+            if (sld_get_box(ctx, box)) fail(sld_last_error(ctx));  // check() precedes scale()
             if (sld_step(ctx, (int)s, 1, obs.data(), &nobs)) fail(sld_last_error(ctx));
-            if (nobs > 0) append_logs(body, obs[0], prm);
+            if (nobs > 0) append_logs(body, obs[0], prm, box);
             std::printf("%lld total time = %.6e dt = %.3e\n", s, sld_total_time(ctx), sld_get_step(ctx));
             ++s;
             if (s % every_config == 0) write_config(ctx, body, (int)((s - 1) / every_config + 1), a, prm.eltemp);
@@ -311,17 +332,21 @@ int main(int argc, char** argv) {
         return 0;
     }
     // the runstep loop (spilady.cpp:43-59): advance to the next configuration
-    // output in one call; every printed step comes back as one record
+    // output in one call; every printed step comes back as one record. With
+    // a barostat the box may change after any step, so one step per call
+    // and its box is logged with its check
     for (long long s = 0; s < nsteps;) {
         long long next = (s / every_config + 1) * every_config;
         if (next > nsteps) next = nsteps;
+        if (barostats) next = s + 1;
         const int n = (int)(next - s);
         obs.resize((size_t)n / every_print + 2);
         int nobs = 0;
+        if (sld_get_box(ctx, box)) fail(sld_last_error(ctx));
         if (sld_step(ctx, (int)s, n, obs.data(), &nobs)) fail(sld_last_error(ctx));
         for (int k = 0; k < nobs; ++k) {
-            append_logs(body, obs[k], prm);
-            std::printf("%d total time = %.6e dt = %.3e\n", obs[k].step, obs[k].total_time, prm.step);
+            append_logs(body, obs[k], prm, box);
+            std::printf("%d total time = %.6e dt = %.3e\n", obs[k].step, obs[k].total_time, sld_get_step(ctx));
         }
         s = next;
         if (s % every_config == 0) write_config(ctx, body, (int)((s - 1) / every_config + 1), a, prm.eltemp);

@@ -92,6 +92,9 @@ def load(path: Optional[str] = None):
     L.sld_total_time.argtypes = [V]
     L.sld_total_time.restype = C.c_double
     L.sld_scale_step.argtypes = [V]
+    L.sld_set_barostat.argtypes = [V, C.POINTER(abi.SldBarostat)]
+    L.sld_scale.argtypes = [V, C.c_int]
+    L.sld_get_box.argtypes = [V, _dp]
     L.sld_set_external.argtypes = [V, _dp, _dp]
     L.sld_get_step.argtypes = [V]
     L.sld_get_step.restype = C.c_double
@@ -284,6 +287,23 @@ class Engine(abi.EngineLike):
     def scale_step(self):
         """scale_step (scale_step_CPU.cpp:33-97)."""
         self._rc(self.L.sld_scale_step(self.h))
+
+    def set_barostat(self, b: abi.SldBarostat):
+        """The STRESS / PRESSURE Berendsen barostats (scale_stress_CPU.cpp,
+        scale_pressure_CPU.cpp); step() applies them after every step."""
+        self._baro = b
+        self._rc(self.L.sld_set_barostat(self.h, C.byref(b)))
+
+    def scale(self, step: int):
+        """scale(step) (scale.cpp:31-44): barostats at their intervals, then
+        changestep; step() calls it itself."""
+        self._rc(self.L.sld_scale(self.h, int(step)))
+
+    def box(self) -> np.ndarray:
+        """The current box matrix (xx yx yy zx zy zz) of the selected replica."""
+        b = np.zeros(6)
+        self._rc(self.L.sld_get_box(self.h, b.ctypes.data_as(_dp)))
+        return b
 
     def set_external(self, hext=None, fext=None):
         """Per-atom external field (eV) and force (eV/A) of the selected

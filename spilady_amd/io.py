@@ -58,6 +58,15 @@ VARIABLE_DEFAULTS: Dict[str, object] = {
     "phi_limit": 0.628319,
     "extfield": 0,             # read extfield.in (tesla) at the start
     "extforce": 0,             # read extforce.in (eV/A) at the start
+    # Berendsen barostats (control.h STRESS / PRESSURE; default.h:177-193)
+    "STRESS": 0,
+    "PRESSURE": 0,
+    "interval_of_scale_stress": 1,
+    "stress_xx": 0e0, "stress_yx": 0e0, "stress_yy": 0e0,   # GPa
+    "stress_zx": 0e0, "stress_zy": 0e0, "stress_zz": 0e0,
+    "interval_of_scale_pressure": 1,
+    "pressure": 0e0,                                          # GPa
+    "baro_damping_time": 1e-13,                               # s
 }
 
 

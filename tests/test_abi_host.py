@@ -34,6 +34,34 @@ def test_ctypes_structs_match_header_sizes():
     assert C.sizeof(abi.SldObs) == 8 + 8 + 8 * 5 + 8 * 4 + 8 * 8 + 8 + 8 * 6 + 8
 
 
+STRUCTS = (("sld_params", abi.SldParams), ("sld_atoms", abi.SldAtoms), ("sld_obs", abi.SldObs),
+           ("sld_cells", abi.SldCells), ("sld_shapes", abi.SldShapes), ("sld_barostat", abi.SldBarostat))
+
+
+def test_ctypes_structs_match_compiled_header(tmp_path):
+    """Every field's offset and every struct's size as gcc lays out
+    include/spilady_hip.h, against the ctypes mirror in spilady_amd/abi.py."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "spilady_hip.h"', "int main(void) {"]
+    for cname, py in STRUCTS:
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'    printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, f, val = ln.split()
+        got[(cname, f)] = int(val)
+    for cname, py in STRUCTS:
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
 def test_plan_matches_reference_groups():
     g = np.load(os.path.join(GOLD, "groups.npz"), allow_pickle=False)
     for nu in g["nus"]:

@@ -148,3 +148,48 @@ def test_driver_reads_external_field_and_force(tmp_path):
     for row_e, row_t, o in zip(enr, tmpl, obs):
         assert list(row_e[2:]) == [fmt(o[k]) for k in ("ave_pe", "ave_ke", "ave_me", "ave_me0", "ave_ee")]
         assert list(row_t[2:]) == [fmt(o[k]) for k in ("tl", "ts_r", "ts_l", "te")]
+
+
+def test_driver_barostats_log_the_moving_box(tmp_path):
+    """STRESS and PRESSURE from variables.in: the box in every str-/prs- line
+    is the box at that check (before the step's scale()), and the last con-
+    file carries the final box and its local volume; all against the
+    in-process engine with the same barostat settings."""
+    from spilady_amd import abi
+    from spilady_amd.engine import Engine
+    tmp = str(tmp_path)
+    _setup(tmp, False, interval_of_print_out=1, STRESS=1, PRESSURE=1, interval_of_scale_stress=2,
+           interval_of_scale_pressure=3, baro_damping_time=2e-14, stress_xx=1.0, stress_yx=0.5,
+           stress_yy=-1.0, stress_zx=0.2, stress_zy=-0.3, stress_zz=2.0, pressure=5.0)
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    st = io.read_con(os.path.join(tmp, "con-start.dat"))
+    v = io.read_variables(os.path.join(tmp, "variables.in"))
+    p = abi.params_from_variables(v, st.box, st.natom)
+    e = Engine(p, st, compute_initial_forces=False)
+    e.set_barostat(abi.barostat_from_variables(v))
+    obs = [e.check(-1)]
+    boxes = [e.box()]
+    for s in range(7):
+        boxes.append(e.box())
+        obs += e.step(s, 1)
+    final_box = e.box()
+    a = e.download()
+    e.close()
+    fmt = lambda x: float("%.15e" % x)  # noqa: E731
+    strl = io.read_log(os.path.join(tmp, "str-drv.dat"))
+    prsl = io.read_log(os.path.join(tmp, "prs-drv.dat"))
+    assert strl.shape == (8, 16) and prsl.shape == (8, 10)
+    for row_st, row_p, o, b in zip(strl, prsl, obs, boxes):
+        b = [float(x) for x in b]
+        dens = st.natom / (b[0] * b[2] * b[5])
+        sv = list(o["ave_stress"])
+        assert list(row_st[2:]) == [fmt(x) for x in b + [dens, sum(sv[:3]) / 3e0] + sv]
+        assert list(row_p[2:]) == [fmt(x) for x in b + [dens, o["pressure"]]]
+    assert np.all(np.abs(final_box - np.asarray(st.box)) > 1e-4)
+    fin = io.read_con(os.path.join(tmp, "con-drv_9999.dat"))
+    np.testing.assert_array_equal(np.asarray(fin.box), np.vectorize(fmt)(final_box))
+    for k in ("r", "p", "s"):
+        np.testing.assert_array_equal(getattr(fin, k), np.vectorize(fmt)(a[k]), err_msg=k)
+    b = final_box
+    assert np.all(fin.local_volume == fmt(b[0] * b[2] * b[5] / st.natom))

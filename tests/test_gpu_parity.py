@@ -209,6 +209,50 @@ def test_changestep_matches_oracle():
     eng.close(); orc.close()
 
 
+def test_barostats_match_oracle():
+    """The stress and pressure barostats with changestep after them, on a
+    compressed lattice (a = 2.80 A, about 13 GPa) with thermostats and the
+    electron temperature: 12 steps, stress every 2 and pressure every 3, each
+    fed by the check() of the step before it. The box after every step and
+    the final state at the tolerance, the time step sequence exactly; a
+    batched context refuses barostats."""
+    from spilady_amd import abi, configs
+    from spilady_amd.engine import BatchEngine, Engine, SldError
+    at = configs.make_bcc_fe(8, 600.0, 5, a=2.80, tilt_max_deg=10.0)
+    v = configs.fe_variables(600.0, 5, thermostats=True, eltemp=True, localcolmot=True)
+    v.update({"STRESS": 1, "PRESSURE": 1, "interval_of_scale_stress": 2, "interval_of_scale_pressure": 3,
+              "baro_damping_time": 2e-14, "stress_xx": 1.0, "stress_yx": 0.5, "stress_yy": -1.0,
+              "stress_zx": 0.2, "stress_zy": -0.3, "stress_zz": 2.0, "pressure": 5.0, "changestep": 1})
+    p = abi.params_from_variables(v, at.box, at.natom)
+    st = consistent_state(p, at)
+    baro = abi.barostat_from_variables(v)
+    eng = Engine(p, st, compute_initial_forces=False)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_PHILOX)
+    eng.set_barostat(baro)
+    orc.set_barostat(baro)
+    dts_e, dts_o = [], []
+    for s in range(12):
+        x = orc.step(s, 1)[0]
+        y = eng.step(s, 1)[0]
+        dts_o.append(orc.dt)
+        dts_e.append(eng.dt)
+        bo, be = orc.box(), eng.box()
+        assert relerr(bo, be) <= TOL, (s, bo, be)
+        for k in ("ave_pe", "ave_ke", "tl", "te", "pressure"):
+            assert abs(x[k] - y[k]) <= TOL * max(abs(x[k]), 1e-300), (s, k, x[k], y[k])
+        assert relerr(x["ave_stress"], y["ave_stress"]) <= TOL, (s, x["ave_stress"], y["ave_stress"])
+    assert dts_e == dts_o
+    assert np.all(np.abs(eng.box() - np.asarray(at.box)) > 1e-3)
+    _compare(eng, orc, "barostats", cells=True)
+    eng.close(); orc.close()
+    vb = dict(v, eltemp=0, localcolmot=0, changestep=0)
+    q = [abi.params_from_variables(vb, at.box, at.natom, replica=r) for r in range(2)]
+    b = BatchEngine(q, [at, at], compute_initial_forces=True)
+    with pytest.raises(SldError):
+        b.set_barostat(baro)
+    b.close()
+
+
 def test_external_field_and_force_match_oracle():
     """extfield and extforce: random per-atom fields (about 10 T) and forces
     (0.1 eV/A), 8 steps with thermostats and the electron temperature; the

@@ -246,3 +246,79 @@ def test_live_reference_external_field_and_force():
     o.close()
     for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0", "heff_h"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_barostats_match_reference_fixture():
+    """barostat_fe1k.npz (the reference built with -DSTRESS -DPRESSURE):
+    12 steps of core(), check() and scale() on a compressed lattice; the
+    box and time step after every step, the str-/prs- rows (box at the
+    check, density, stresses, pressure) and the final r, p, s bit for bit."""
+    g = _load("barostat_fe1k.npz")
+    v = _variables(g)
+    n = g["in_r"].shape[0]
+    st = {k: g[f"in_{k}"] for k in ("r", "p", "f", "s", "stress", "rho", "ke", "pe", "me", "me0")}
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), g["box"], n)
+    p.changestep = 1  # compiled into the shipped control.h
+    o = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_MWC)
+    o.set_barostat(abi.barostat_from_variables(v))
+    fmt = lambda x: float("%.15e" % x)  # noqa: E731
+    box = o.box()
+    for cur in range(12):
+        ob = o.step(cur, 1)[0]
+        srow, prow = g["str"][cur + 1], g["prs"][cur + 1]
+        assert int(srow[0]) == cur and srow[1] == fmt(ob["total_time"])
+        sv = list(ob["ave_stress"])
+        dens = n / (box[0] * box[2] * box[5])
+        assert list(srow[2:]) == [fmt(x) for x in list(box) + [dens, sum(sv[:3]) / 3e0] + sv], cur
+        assert list(prow[2:]) == [fmt(x) for x in list(box) + [dens, ob["pressure"]]], cur
+        box = o.box()
+        np.testing.assert_array_equal(box, g["boxes"][cur])
+        assert o.dt == g["dts"][cur]
+    b = o.download()
+    o.close()
+    for k in ("r", "p", "s"):
+        np.testing.assert_array_equal(b[k], g[f"final_{k}"], err_msg=k)
+
+
+@pytest.mark.skipif(not refpy.available_baro(),
+                    reason="reference with STRESS/PRESSURE not built (oracle/Makefile.ref OUT=_ref/baro)")
+def test_live_reference_barostats():
+    """The Berendsen barostats (scale_stress_CPU.cpp:33-86, scale_pressure_CPU.cpp:
+    33-61) on a compressed lattice (a = 2.80 A, about 13 GPa), stress every 2
+    steps and pressure every 3, with check() after every step as their input
+    and the shipped changestep after them (scale.cpp:31-44): 12 steps, the box
+    matrix after every step and the final state bit for bit. Thermostats are
+    off: the reference build renormalises the electron energy in check()."""
+    at = configs.make_bcc_fe(8, 300.0, 5, a=2.80, tilt_max_deg=10.0)
+    v = configs.fe_variables(300.0, 5, thermostats=False)
+    v.update({"STRESS": 1, "PRESSURE": 1, "interval_of_scale_stress": 2, "interval_of_scale_pressure": 3,
+              "baro_damping_time": 2e-14, "stress_xx": 1.0, "stress_yx": 0.5, "stress_yy": -1.0,
+              "stress_zx": 0.2, "stress_zy": -0.3, "stress_zz": 2.0, "pressure": 5.0})
+    st = refpy.consistent_start(v, at)
+    d = tempfile.mkdtemp()
+    ref = refpy.Reference(refpy.REF_LIB_BARO)
+    ref.init_memory(d, v, st)
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), st.box, st.natom)
+    p.changestep = 1  # compiled into the shipped control.h
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    o.set_barostat(abi.barostat_from_variables(v))
+    boxes_r, boxes_o = [], []
+    for cur in range(12):
+        ref.run(cur, 1, True)
+        ref.lib.ref_scale(cur)
+        boxes_r.append(ref.box())
+        o.step(cur, 1)
+        boxes_o.append(o.box())
+    a = ref.atoms()
+    dt_r = ref.ref_step()
+    ref.free()
+    b = o.download()
+    dt_o = o.dt
+    o.close()
+    np.testing.assert_array_equal(np.array(boxes_r), np.array(boxes_o))
+    assert dt_r == dt_o
+    # both barostats moved the box: the diagonal and the tilts changed
+    bx = np.array(boxes_o)
+    assert np.all(np.abs(bx[-1] - st.box) > 1e-3)
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
