@@ -67,6 +67,11 @@ typedef struct sld_params {
     int32_t extfield;          /* control.h: per-atom external field (external_field_CPU.cpp), zero
                                   until set with sld_set_external */
     int32_t extforce;          /* control.h: per-atom external force (external_force_CPU.cpp), same */
+    int32_t quantumnoise;      /* control.h: quantum thermostat, coloured lattice noise
+                                  (quantum_noise_CPU.cpp); needs lattlang */
+    int32_t msteps_quantum;    /* the noise is redrawn every this many core_dp calls (default.h:223) */
+    int32_t nfrequency_quantum;/* frequency intervals; the filter has 2x this many taps (default.h:224) */
+    int32_t reserved1;
 } sld_params;
 
 /* Host view of the per-atom state (struct.h:44-120, con- columns of

@@ -81,6 +81,7 @@ def lib():
         L.orc_set_barostat.argtypes = [C.c_void_p, C.POINTER(abi.SldBarostat)]
         L.orc_scale.argtypes = [C.c_void_p, C.c_int]
         L.orc_get_box.argtypes = [C.c_void_p, _dp]
+        L.orc_quantum_H.argtypes = [C.c_void_p, _dp]
         L.orc_set_external.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_get_step.argtypes = [C.c_void_p]
         L.orc_get_step.restype = C.c_double
@@ -212,6 +213,13 @@ class Oracle(abi.EngineLike):
         b = np.zeros(6)
         self.L.orc_get_box(self.h, b.ctypes.data_as(_dp))
         return b
+
+    def quantum_H(self) -> np.ndarray:
+        """The quantum thermostat's filter (empty without quantumnoise)."""
+        h = np.zeros(self.L.orc_quantum_H(self.h, None))
+        if h.size:
+            self.L.orc_quantum_H(self.h, h.ctypes.data_as(_dp))
+        return h
 
     def set_external(self, hext=None, fext=None):
         """Per-atom external field (eV) and force (eV/A), [natom][3]."""

@@ -136,6 +136,10 @@ int ref_init_memory(const char* dir, int n, const double* dbox, const double* r,
     last_total_time_stress = total_time;
 #endif
     initialize_random_number();
+    // initial.cpp:102-104: the quantum thermostat's filter and noise history
+#ifdef quantumnoise
+    initial_quantum_noise();
+#endif
     check(-1);
     g_live = true;
     return natom;
@@ -167,6 +171,21 @@ void ref_check(int cur) { Quiet q(true); check(cur); }
 // scale() (scale.cpp:31-44): with the shipped control.h, scale_step only;
 // a build with -DSTRESS -DPRESSURE adds the two barostats before it
 void ref_scale(int cur) { Quiet q(true); scale(cur); }
+// the quantum thermostat's filter H (quantum_noise_CPU.cpp:83-113), 0 without it
+int ref_quantum_taps(void) {
+#ifdef quantumnoise
+    return Nfrequency_quantum_2;
+#else
+    return 0;
+#endif
+}
+void ref_quantum_H(double* out) {
+#ifdef quantumnoise
+    for (int i = 0; i < Nfrequency_quantum_2; ++i) out[i] = H_ptr[i];
+#else
+    (void)out;
+#endif
+}
 void ref_get_box(double* b) { b[0] = d.xx; b[1] = d.yx; b[2] = d.yy; b[3] = d.zx; b[4] = d.zy; b[5] = d.zz; }
 
 int ref_natom(void) { return natom; }

@@ -30,7 +30,7 @@ REF_KEYS = ["OMP_threads", "out_body", "step", "interval_of_print_out",
             "min_length_link_cell", "displace_limit", "phi_limit",
             "interval_of_scale_stress", "stress_xx", "stress_yx", "stress_yy",
             "stress_zx", "stress_zy", "stress_zz", "interval_of_scale_pressure",
-            "pressure", "baro_damping_time"]
+            "pressure", "baro_damping_time", "Msteps_quantum", "Nfrequency_quantum"]
 
 # Fixed switches of the reference build (shipped control.h).
 REF_SWITCHES = {"spinlang": 1, "lattlang": 1, "eltemp": 1, "localcolmot": 1,
@@ -57,6 +57,14 @@ def available_baro() -> bool:
     return os.path.exists(REF_LIB_BARO)
 
 
+# the same sources built with -Dquantumnoise (Makefile.ref OUT=_ref/qn)
+REF_LIB_QN = os.path.join(HERE, "_ref", "qn", "libspilady_ref.so")
+
+
+def available_qn() -> bool:
+    return os.path.exists(REF_LIB_QN)
+
+
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
@@ -76,6 +84,7 @@ class Reference:
         L.ref_check.argtypes = [C.c_int]
         L.ref_scale.argtypes = [C.c_int]
         L.ref_get_box.argtypes = [_dp]
+        L.ref_quantum_H.argtypes = [_dp]
         for nm in ("ref_step", "ref_total_time", "ref_virial", "ref_initial_ave_energy"):
             getattr(L, nm).restype = C.c_double
         L.ref_scalar.restype = C.c_double
@@ -185,6 +194,12 @@ class Reference:
         b = np.zeros(6)
         self.lib.ref_get_box(_ptr(b))
         return b
+
+    def quantum_H(self) -> np.ndarray:
+        h = np.zeros(self.lib.ref_quantum_taps())
+        if h.size:
+            self.lib.ref_quantum_H(_ptr(h))
+        return h
 
     def scalar(self, name: str) -> float:
         return self.lib.ref_scalar(name.encode())

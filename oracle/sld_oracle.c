@@ -21,6 +21,7 @@
 /* spilady.h:53-62 */
 #define BOLTZ 8.617343e-5
 #define HBAR 6.58211899e-16
+#define PI 3.141592653589793238462e0   /* Pi_num (spilady.h:61) */
 #define EL_G 2.0023193043622e0
 #define EVTOJ 1.602176487e-19
 
@@ -64,6 +65,11 @@ struct orc_ctx {
     sld_barostat baro;
     double last_t_stress, last_t_pressure;
     double last_stress[6], last_pressure;
+    /* quantum thermostat (quantum_noise_CPU.cpp): filter, per-DOF history
+     * [3N][nf2] oldest first, current noise and call count per DOF */
+    int qn_nf2;
+    double *qH, *qmem, *qval;
+    int* qcount;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -370,6 +376,101 @@ static double mwc_normal(orc_ctx* o) {
     o->rand1 = -2e0 * log(o->rand1);
     o->rand2 = mwc_uniform(o) * 6.283185307179586;
     return sqrt(o->rand1) * cos(o->rand2);
+}
+
+/* The counter-based stream of the quantum noise history: element e of an
+ * atom's three histories is normal3 at call QN_CALL0 + e, a range no lattice
+ * or spin pass reaches. */
+#define QN_CALL0 (1ull << 62)
+
+static inline void noise3(orc_ctx* o, int atom, double out[3]);
+
+/* initial_quantum_noise (quantum_noise_CPU.cpp:83-130): the filter H is the
+ * cosine transform of H~(w) = sqrt(hbar|w| (1/2 + 1/expm1(hbar|w|/T)))
+ * * (w h/2) / sin(w h/2) on 2 Nf frequencies up to pi/h, h = M dt / 2
+ * (H~(0) = sqrt(T)); every history starts with fresh normals. */
+static void quantum_init(orc_ctx* o) {
+    const int Nf = o->prm.nfrequency_quantum, Nf2 = 2 * Nf, n = 3 * o->natom;
+    const double T = o->prm.temperature;
+    const double h = o->prm.msteps_quantum * o->prm.step / 2e0;
+    const double max_frequency = PI / h;
+    const double delta_frequency = max_frequency / (double)Nf;
+    o->qn_nf2 = Nf2;
+    o->qH = (double*)calloc((size_t)Nf2, sizeof(double));
+    double* Ht = (double*)calloc((size_t)Nf2, sizeof(double));
+    for (int i = 0; i < Nf2; ++i) {
+        int k = i - Nf;
+        double frequency = delta_frequency * (double)k;
+        Ht[i] = sqrt(HBAR * fabs(frequency) * (0.5e0 + 1e0 / expm1(HBAR * fabs(frequency) / T)));
+        Ht[i] *= (frequency * h / 2e0) / sin(frequency * h / 2e0);
+        if (k == 0) Ht[i] = sqrt(T);
+    }
+    for (int i = 0; i < Nf2; ++i) {
+        o->qH[i] = 0e0;
+        for (int j = 0; j < Nf2; ++j) {
+            int k = j - Nf;
+            o->qH[i] += Ht[j] * cos(PI / (double)Nf * (double)k * (double)(i - Nf));
+        }
+        o->qH[i] /= (double)Nf2;
+    }
+    free(Ht);
+    o->qmem = (double*)calloc((size_t)n * Nf2, sizeof(double));
+    o->qval = (double*)calloc((size_t)n, sizeof(double));
+    o->qcount = (int*)calloc((size_t)n, sizeof(int));
+    if (o->rng_mode == ORC_RNG_MWC) {
+        for (size_t i = 0; i < (size_t)n * Nf2; ++i) o->qmem[i] = mwc_normal(o);
+    } else {
+        for (int a = 0; a < o->natom; ++a)
+            for (int j = 0; j < Nf2; ++j) {
+                double z[3];
+                orc_normal3(o->prm.random_seed, o->prm.replica, a, QN_CALL0 + (uint64_t)j, z);
+                for (int k = 0; k < 3; ++k) o->qmem[(size_t)(3 * a + k) * Nf2 + j] = z[k];
+            }
+    }
+}
+
+/* quantum_noise (quantum_noise_CPU.cpp:60-81): every M-th call of a degree
+ * of freedom, its noise becomes the filter applied to its history (newest
+ * against H[0]), the history shifts by one and takes a fresh normal. */
+static double quantum_noise(orc_ctx* o, int atom, int k) {
+    const int Nf2 = o->qn_nf2, n = 3 * atom + k;
+    double* mem = o->qmem + (size_t)n * Nf2;
+    if (o->qcount[n] % o->prm.msteps_quantum == 0) {
+        double noise = 0e0;
+        for (int i = 0; i < Nf2; ++i) noise += o->qH[i] * mem[Nf2 - i - 1];
+        for (int i = 0; i < Nf2 - 1; ++i) mem[i] = mem[i + 1];
+        if (o->rng_mode == ORC_RNG_MWC) {
+            mem[Nf2 - 1] = mwc_normal(o);
+        } else {
+            /* the r-th refresh appends history element Nf2 + r */
+            uint64_t e = (uint64_t)Nf2 + (uint64_t)(o->qcount[n] / o->prm.msteps_quantum);
+            double z[3];
+            orc_normal3(o->prm.random_seed, o->prm.replica, atom, QN_CALL0 + e, z);
+            mem[Nf2 - 1] = z[k];
+        }
+        o->qval[n] = noise;
+    }
+    ++o->qcount[n];
+    return o->qval[n];
+}
+
+/* the random force of one atom in core_dp_A (core_dp_CPU.cpp:134-155):
+ * white noise of variance 2 T gamma / dt, or with quantumnoise the coloured
+ * noise scaled by sqrt(2 gamma / h), h = M dt */
+static vec fluct_force(orc_ctx* o, int atom, double T, double gl, double dt) {
+    double nz[3], len;
+    if (o->prm.quantumnoise) {
+        double h = dt * o->prm.msteps_quantum;
+        len = sqrt(2e0 * gl / h);
+        nz[0] = quantum_noise(o, atom, 0);
+        nz[1] = quantum_noise(o, atom, 1);
+        nz[2] = quantum_noise(o, atom, 2);
+    } else {
+        len = sqrt(2e0 * T * gl / dt);
+        noise3(o, atom, nz);
+    }
+    vec ff = {nz[0], nz[1], nz[2]};
+    return v_times(len, ff);
 }
 
 static inline void noise3(orc_ctx* o, int atom, double out[3]) {
@@ -881,11 +982,7 @@ static void core_dp_A(orc_ctx* o, double dt) {
                 if (a != -1) { o->ave_fluct[c] = v_zero(); active = 1; }
                 for (; a != -1; a = o->next[a]) {
                     double T = o->prm.eltemp ? o->Te[c] : o->prm.temperature;
-                    double len = sqrt(2e0 * T * gl / dt);
-                    double nz[3];
-                    noise3(o, a, nz);
-                    vec ff = {nz[0], nz[1], nz[2]};
-                    ff = v_times(len, ff);
+                    vec ff = fluct_force(o, a, T, gl, dt);
                     o->p[a] = v_add(o->p[a], v_times(dt, v_add(dp_force(o, a), ff)));
                     o->ave_fluct[c] = v_add(o->ave_fluct[c], ff);
                 }
@@ -896,11 +993,7 @@ static void core_dp_A(orc_ctx* o, double dt) {
     } else {
         for (int i = 0; i < o->natom; ++i) {
             double T = o->prm.eltemp ? o->Te[o->cell_new[i]] : o->prm.temperature;
-            double len = sqrt(2e0 * T * gl / dt);
-            double nz[3];
-            noise3(o, i, nz);
-            vec ff = {nz[0], nz[1], nz[2]};
-            ff = v_times(len, ff);
+            vec ff = fluct_force(o, i, T, gl, dt);
             o->p[i] = v_add(o->p[i], v_times(dt, v_add(dp_force(o, i), ff)));
         }
     }
@@ -1295,6 +1388,13 @@ void orc_scale(orc_ctx* o, int step) {
 
 void orc_get_box(const orc_ctx* o, double* box) { memcpy(box, o->d, sizeof(o->d)); }
 
+/* the quantum thermostat's filter (2 nfrequency_quantum taps); 0 without it */
+int orc_quantum_H(const orc_ctx* o, double* out) {
+    if (!o->qH) return 0;
+    if (out) memcpy(out, o->qH, sizeof(double) * o->qn_nf2);
+    return o->qn_nf2;
+}
+
 /* spilady.cpp:43-57: core, time, check on printed steps, scale */
 int orc_step(orc_ctx* o, int first_step, int nsteps, sld_obs* obs) {
     int k = 0;
@@ -1385,6 +1485,7 @@ orc_ctx* orc_create(const sld_params* p, const sld_atoms* in, int compute_initia
     o->m_z = (uint32_t)(0 * 3 + (int)p->random_seed * 2);
     o->spare = 0;
     o->noise_call = 0;
+    if (o->prm.quantumnoise) quantum_init(o);  /* initial.cpp:102-104 */
     orc_check(o, -1, NULL);
     return o;
 }
@@ -1397,7 +1498,7 @@ void orc_destroy(orc_ctx* o) {
                     o->prev, o->cell_old, o->cell_new, o->head, o->tail, o->count, o->neigh, o->type,
                     o->group_size, o->group_members, o->Te, o->Ee, o->Tl, o->Gel, o->Ges, o->Ts_R,
                     o->Ts_L, o->sum_R_up, o->sum_R_dn, o->sum_L_up, o->sum_L_dn, o->ave_p, o->ave_fluct,
-                    o->hext, o->fext};
+                    o->hext, o->fext, o->qH, o->qmem, o->qval, o->qcount};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
     free(o);
 }

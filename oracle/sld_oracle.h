@@ -51,6 +51,9 @@ double orc_get_step(const orc_ctx* o);
 void orc_set_barostat(orc_ctx* o, const sld_barostat* b);
 void orc_scale(orc_ctx* o, int step);
 void orc_get_box(const orc_ctx* o, double* box);
+/* the quantum thermostat's filter H (quantum_noise_CPU.cpp:83-113) into out
+ * (may be NULL); returns its length, 0 without quantumnoise */
+int orc_quantum_H(const orc_ctx* o, double* out);
 
 void orc_download(const orc_ctx* o, sld_atoms* out);
 void orc_upload(orc_ctx* o, const sld_atoms* in);

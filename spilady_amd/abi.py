@@ -41,6 +41,10 @@ class SldParams(C.Structure):
         ("phi_limit", C.c_double),
         ("extfield", C.c_int32),
         ("extforce", C.c_int32),
+        ("quantumnoise", C.c_int32),
+        ("msteps_quantum", C.c_int32),
+        ("nfrequency_quantum", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 
@@ -210,6 +214,10 @@ def params_from_variables(v: Dict[str, object], box, natom: int, replica: int = 
     p.phi_limit = float(v.get("phi_limit", 2.0 * np.pi / 10.0))
     p.extfield = int(v.get("extfield", 0))
     p.extforce = int(v.get("extforce", 0))
+    # quantumnoise (control.h) and its sizes (default.h:222-225, read_variables.cpp:245-250)
+    p.quantumnoise = int(v.get("quantumnoise", 0))
+    p.msteps_quantum = int(v.get("Msteps_quantum", 50))
+    p.nfrequency_quantum = int(v.get("Nfrequency_quantum", 150))
     return p
 
 

@@ -85,6 +85,7 @@ struct sld_ctx {
     double total_time = 0.0;
     uint64_t pass = 0;                       // noise pass counter
     uint64_t pass_dev = 0;                   // value of the device counter D.pass_base
+    uint64_t qcalls = 0;                     // core_dp_A calls so far (quantum thermostat)
     // barostats and the last check()'s stresses and pressure they act on
     sld_barostat baro{};
     double last_t_stress = 0.0, last_t_pressure = 0.0;
@@ -372,6 +373,13 @@ int do_ds(sld_ctx* c, double dt) {
 // before and after centring
 int do_dp(sld_ctx* c, double dt) {
     int nb = nblk(c->D.C, 8);
+    // quantum_noise (quantum_noise_CPU.cpp:60-81): every M-th core_dp_A call
+    // redraws the noise of all degrees of freedom at once
+    if (c->phys.quantumnoise && c->phys.lattlang) {
+        if (c->qcalls % (uint64_t)c->phys.qm == 0)
+            LAUNCH(c, k_qn_refresh, nblk(c->D.N, 256), 256, c->phys, c->D, c->qcalls / (uint64_t)c->phys.qm);
+        ++c->qcalls;
+    }
     LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, (int)(c->pass - c->pass_dev), c->d_part);
     if (c->phys.lattlang) c->pass++;
     TRY(reduce_seg(c, c->d_cell8_seg, 3, 0));
@@ -710,7 +718,7 @@ void drop_graphs(sld_ctx* c) {
 // Capturing runs the host logic of the step (buffer swap, pass counting)
 // once; a replay applies the same host effects.
 int run_step(sld_ctx* c) {
-    if (c->graph_mode < 2 || !c->timed_name.empty() || c->prm.changestep) return do_core(c);
+    if (c->graph_mode < 2 || !c->timed_name.empty() || c->prm.changestep || c->phys.quantumnoise) return do_core(c);
     TRY(ensure_nb(c));  // a step starts with current lists
     const int parity = c->D.cl == c->cl_first ? 0 : 1;
     hipGraphExec_t& ex = c->step_exec[parity];
@@ -857,7 +865,34 @@ bool same_shared(const sld_params& a, const sld_params& b) {
            a.interval_of_print_out == b.interval_of_print_out && a.spinlang == b.spinlang &&
            a.lattlang == b.lattlang && a.eltemp == b.eltemp && a.localcolmot == b.localcolmot &&
            a.renormalize_energy == b.renormalize_energy && a.changestep == b.changestep &&
-           a.extfield == b.extfield && a.extforce == b.extforce;
+           a.extfield == b.extfield && a.extforce == b.extforce && a.quantumnoise == b.quantumnoise &&
+           (!a.quantumnoise ||
+            (a.msteps_quantum == b.msteps_quantum && a.nfrequency_quantum == b.nfrequency_quantum));
+}
+
+// initial_quantum_noise (quantum_noise_CPU.cpp:83-113): the filter H is the
+// cosine transform of H~(w) = sqrt(hbar|w| (1/2 + 1/expm1(hbar|w|/T)))
+// * (w h/2) / sin(w h/2) on 2 Nf frequencies up to pi/h, h = M dt / 2,
+// with H~(0) = sqrt(T); the same host arithmetic in the same order.
+std::vector<double> quantum_filter(const sld_params& P) {
+    const int Nf = P.nfrequency_quantum, Nf2 = 2 * Nf;
+    const double T = P.temperature;
+    const double h = P.msteps_quantum * P.step / 2e0;
+    const double pi = 3.141592653589793238462e0;
+    const double delta_frequency = pi / h / (double)Nf;
+    std::vector<double> Ht(Nf2), H(Nf2, 0.0);
+    for (int i = 0; i < Nf2; ++i) {
+        const int k = i - Nf;
+        const double frequency = delta_frequency * (double)k;
+        Ht[i] = std::sqrt(kHbar * std::fabs(frequency) * (0.5e0 + 1e0 / std::expm1(kHbar * std::fabs(frequency) / T)));
+        Ht[i] *= (frequency * h / 2e0) / std::sin(frequency * h / 2e0);
+        if (k == 0) Ht[i] = std::sqrt(T);
+    }
+    for (int i = 0; i < Nf2; ++i) {
+        for (int j = 0; j < Nf2; ++j) H[i] += Ht[j] * std::cos(pi / (double)Nf * (double)(j - Nf) * (double)(i - Nf));
+        H[i] /= (double)Nf2;
+    }
+    return H;
 }
 
 int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms* init, int compute_forces) {
@@ -878,6 +913,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
         if (P.localcolmot && !P.eltemp) P.localcolmot = 0;
         if (P.renormalize_energy && !P.eltemp) P.renormalize_energy = 0;
         if (P.step <= 0) return set_err(c, SLD_ERR_ARG, "step must be positive");
+        if (P.quantumnoise && (P.msteps_quantum < 1 || P.nfrequency_quantum < 1))
+            return set_err(c, SLD_ERR_ARG, "quantumnoise needs msteps_quantum >= 1 and nfrequency_quantum >= 1");
         if (r > 0 && !same_shared(P, c->rh[0].prm))
             return set_err(c, SLD_ERR_ARG, "replica %d: dt, mass, damping, cutoffs, print interval and switches "
                                            "must equal replica 0's", r);
@@ -959,6 +996,9 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     ph.localcolmot = P0.localcolmot;
     ph.extfield = P0.extfield;
     ph.extforce = P0.extforce;
+    ph.quantumnoise = P0.quantumnoise;
+    ph.qm = P0.msteps_quantum;
+    ph.qnf2 = 2 * P0.nfrequency_quantum;
 
     // tables (shared)
     std::shared_ptr<const HostTables> ht = shared_tables(P0.rcut_max);
@@ -998,6 +1038,20 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     TRY(dalloc(c, &D.zn, N));
     if (P0.extfield) TRY(dalloc(c, &D.hext, N));  // zero: the reference's default field
     if (P0.extforce) TRY(dalloc(c, &D.fext, N));
+    if (P0.quantumnoise) {
+        const size_t nf2 = (size_t)ph.qnf2;
+        TRY(dalloc(c, &D.qring, nf2 * 3 * (size_t)N));
+        TRY(dalloc(c, &D.qval, 3 * (size_t)N));
+        std::vector<double> H;
+        for (int r = 0; r < nrep; ++r) {
+            const std::vector<double> h = quantum_filter(c->rh[r].prm);
+            H.insert(H.end(), h.begin(), h.end());
+        }
+        double* d_qH = nullptr;
+        TRY(upload_vec(c, &d_qH, H));
+        D.qH = d_qH;
+        LAUNCH(c, k_qn_init, nblk(N, 256), 256, ph, D);
+    }
     TRY(dalloc(c, &D.pass_base, 1));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
@@ -1462,7 +1516,7 @@ struct Snapshot {
     void* buf = nullptr;
     Dev D{};
     int *cl2 = nullptr, *cn2 = nullptr;
-    uint64_t pass = 0, pass_dev = 0;
+    uint64_t pass = 0, pass_dev = 0, qcalls = 0;
     double total_time = 0.0;
     bool nb_dirty = true;
 };
@@ -1483,6 +1537,7 @@ int snapshot_take(sld_ctx* c, Snapshot& s) {
     s.cn2 = c->d_cn2;
     s.pass = c->pass;
     s.pass_dev = c->pass_dev;
+    s.qcalls = c->qcalls;
     s.total_time = c->total_time;
     s.nb_dirty = c->nb_dirty;
     return SLD_OK;
@@ -1505,6 +1560,7 @@ int snapshot_restore(sld_ctx* c, Snapshot& s) {
     c->d_cn2 = s.cn2;
     c->pass = s.pass;
     c->pass_dev = s.pass_dev;
+    c->qcalls = s.qcalls;
     c->total_time = s.total_time;
     c->nb_dirty = s.nb_dirty;
     return SLD_OK;

@@ -36,6 +36,11 @@ struct Dev {
     double4* zn;               // Langevin normals of the current spin pass [N]
     double4 *hext, *fext;      // external field (eV) and force (eV/A) [N]; null when off
     uint64_t* pass_base;       // noise passes consumed before the current step [1]
+    // quantum thermostat: noise history ring [2 Nf][3N] (slot e % 2Nf holds
+    // history element e), current noise [3N], filter of each replica [R][2 Nf];
+    // null when off
+    double *qring, *qval;
+    const double* qH;
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
     double* stress;            // [6][N]: 11 22 33 12 23 31
     int* cell;                 // link cell of each atom (new_cell_index)
@@ -57,6 +62,7 @@ struct Phys {
     double rcut_pot, rcut_mag, rcut_pot_sq, rcut_mag_sq, rcut_max_sq;
     int spinlang, lattlang, eltemp, localcolmot;
     int extfield, extforce;
+    int quantumnoise, qm, qnf2;  // quantum thermostat: on, redraw period M, filter taps 2 Nf
 };
 
 // per-replica block of reduced scalars
@@ -74,6 +80,8 @@ __global__ void k_scale_r(Dev D, double factor);
 __global__ void k_rebox_r(Dev D, Box old_box, Box new_box);
 __global__ void k_reduce_max2(const double* partials, int nblk, double* out);
 __global__ void k_pass_advance(Dev D, int n);
+__global__ void k_qn_init(Phys P, Dev D);
+__global__ void k_qn_refresh(Phys P, Dev D, uint64_t r);
 __global__ void k_spin_phase(const int* cells, int ncell, int backward, double dt, Tables T, Phys P,
                              Dev D);
 __global__ void k_dp_cells(double dt, Phys P, Dev D, int pass_off, double* partials);

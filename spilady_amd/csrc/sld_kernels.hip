@@ -740,6 +740,62 @@ __device__ __forceinline__ double3 dp_force(const Phys& P, const Dev& D, int a) 
     return f;
 }
 
+// ---------------------------------------------------------------------------
+// quantum thermostat (quantum_noise_CPU.cpp). History element e of an atom's
+// three degrees of freedom is normal3 at pass kQnCall0 + e, a range that no
+// lattice or spin pass reaches; element e lives in ring slot e % 2Nf.
+constexpr uint64_t kQnCall0 = 1ull << 62;
+
+// initial_quantum_noise (:115-119): elements 0 .. 2Nf-1, one thread per atom
+__global__ __launch_bounds__(256) void k_qn_init(Phys P, Dev D) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.N || D.arep[a] < 0) return;
+    const Rep& R = D.rep[D.arep[a]];
+    const size_t stride = 3 * (size_t)D.N;
+    for (int j = 0; j < P.qnf2; ++j) {
+        const double3 z = normal3(R.seed, R.replica, a - R.atom0, kQnCall0 + (uint64_t)j);
+        double* m = D.qring + (size_t)j * stride + 3 * (size_t)a;
+        m[0] = z.x;
+        m[1] = z.y;
+        m[2] = z.z;
+    }
+}
+
+// quantum_noise (:60-81) for the r-th redraw of every degree of freedom at
+// once: the noise is sum_i H[i] * element (r + 2Nf - 1 - i), newest first;
+// element 2Nf + r then replaces element r, the oldest, in its slot. Each
+// lane reads its atom's three doubles of a slot, so a wave reads 1.5 KB
+// contiguous per tap.
+__global__ __launch_bounds__(256) void k_qn_refresh(Phys P, Dev D, uint64_t r) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.N || D.arep[a] < 0) return;
+    const int rr = D.arep[a];
+    const Rep& R = D.rep[rr];
+    const int nf2 = P.qnf2;
+    const double* H = D.qH + (size_t)rr * nf2;
+    const size_t stride = 3 * (size_t)D.N;
+    const double* base = D.qring + 3 * (size_t)a;
+    int slot = (int)((r + (uint64_t)nf2 - 1) % (uint64_t)nf2);
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    for (int i = 0; i < nf2; ++i) {
+        const double* m = base + (size_t)slot * stride;
+        const double h = H[i];
+        n0 += h * m[0];
+        n1 += h * m[1];
+        n2 += h * m[2];
+        slot = slot == 0 ? nf2 - 1 : slot - 1;
+    }
+    const double3 z = normal3(R.seed, R.replica, a - R.atom0, kQnCall0 + (uint64_t)nf2 + r);
+    double* m = D.qring + (size_t)(r % (uint64_t)nf2) * stride + 3 * (size_t)a;
+    m[0] = z.x;
+    m[1] = z.y;
+    m[2] = z.z;
+    double* q = D.qval + 3 * (size_t)a;
+    q[0] = n0;
+    q[1] = n1;
+    q[2] = n2;
+}
+
 // One half-wave (32 lanes) per link cell, one atom per lane.
 __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int pass_off,
                                                   double* partials) {
@@ -760,11 +816,19 @@ __global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int 
             p = mul3(eb, p);
             const Rep& R = D.rep[rr];
             const double Tk = P.eltemp ? D.Te[c] : R.temperature;
-            const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
             double3 ff = make_double3(0.0, 0.0, 0.0);
             if (has) {
-                const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
-                ff = mul3(len, normal3(R.seed, R.replica, a - R.atom0, pass));
+                if (P.quantumnoise) {
+                    // core_dp_CPU.cpp:137-146: coloured noise scaled by sqrt(2 gamma / h), h = M dt
+                    const double hq = dt * P.qm;
+                    const double len = sqrt(2.0 * P.gamma_L / hq);
+                    const double* q = D.qval + 3 * (size_t)a;
+                    ff = mul3(len, make_double3(q[0], q[1], q[2]));
+                } else {
+                    const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
+                    const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
+                    ff = mul3(len, normal3(R.seed, R.replica, a - R.atom0, pass));
+                }
                 p = add3(p, mul3(dt, add3(dp_force(P, D, a), ff)));
             }
             if (P.localcolmot && n > 0) {

@@ -56,7 +56,9 @@ struct Vars {
         // Berendsen barostats: control.h STRESS / PRESSURE, default.h:177-193
         {"STRESS", "0"}, {"PRESSURE", "0"}, {"interval_of_scale_stress", "1"}, {"stress_xx", "0"},
         {"stress_yx", "0"}, {"stress_yy", "0"}, {"stress_zx", "0"}, {"stress_zy", "0"}, {"stress_zz", "0"},
-        {"interval_of_scale_pressure", "1"}, {"pressure", "0"}, {"baro_damping_time", "1e-13"}};
+        {"interval_of_scale_pressure", "1"}, {"pressure", "0"}, {"baro_damping_time", "1e-13"},
+        // quantum thermostat: control.h quantumnoise, default.h:222-225
+        {"quantumnoise", "0"}, {"Msteps_quantum", "50"}, {"Nfrequency_quantum", "150"}};
     void read(const char* path) {
         std::ifstream in(path);
         if (!in) fail(std::string("cannot open ") + path);
@@ -251,6 +253,9 @@ int main(int argc, char** argv) {
     prm.extforce = (int)v.i("extforce");
     prm.displace_limit = v.d("displace_limit");
     prm.phi_limit = v.d("phi_limit");
+    prm.quantumnoise = (int)v.i("quantumnoise");
+    prm.msteps_quantum = (int)v.i("Msteps_quantum");
+    prm.nfrequency_quantum = (int)v.i("Nfrequency_quantum");
     sld_barostat baro = {};
     baro.stress_on = (int)v.i("STRESS");
     baro.pressure_on = (int)v.i("PRESSURE");

@@ -67,6 +67,10 @@ VARIABLE_DEFAULTS: Dict[str, object] = {
     "interval_of_scale_pressure": 1,
     "pressure": 0e0,                                          # GPa
     "baro_damping_time": 1e-13,                               # s
+    # quantum thermostat (control.h quantumnoise; default.h:222-225)
+    "quantumnoise": 0,
+    "Msteps_quantum": 50,
+    "Nfrequency_quantum": 150,
 }
 
 

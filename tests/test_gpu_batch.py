@@ -71,3 +71,27 @@ def test_batch_replica_matches_oracle():
         assert_close(x[f], y[f], f"replica 1 {f}", TOL)
     o.close()
     b.close()
+
+
+def test_batch_quantum_noise_matches_oracle():
+    """The quantum thermostat in a batch: each replica's filter follows its
+    own temperature (300, 900 and 1,400 K), and its coloured noise its own
+    seed and stream id. M = 3 and Nf = 8, 8 steps (six redraws); every
+    replica against an oracle run of it alone, at the parity tolerance."""
+    from spilady_amd.engine import BatchEngine
+    reps = []
+    for r, (nu, T, seed) in enumerate(REPLICAS):
+        v, p, at = fe_params(nu, T, seed, thermostats=True, replica=r)
+        p.quantumnoise, p.msteps_quantum, p.nfrequency_quantum = 1, 3, 8
+        reps.append((p, consistent_state(p, at)))
+    b = BatchEngine([p for p, _ in reps], [st for _, st in reps])
+    b.step(0, 8)
+    for r, (p, st) in enumerate(reps):
+        o = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_PHILOX)
+        o.step(0, 8)
+        b.select(r)
+        x, y = o.download(), b.download()
+        for f in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+            assert_close(x[f], y[f], f"replica {r} {f}", TOL)
+        o.close()
+    b.close()

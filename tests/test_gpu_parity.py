@@ -253,6 +253,27 @@ def test_barostats_match_oracle():
     b.close()
 
 
+def test_quantum_noise_matches_oracle():
+    """The quantum thermostat (coloured lattice noise from a per-degree-of-
+    freedom history filtered by H) with M = 3 and Nf = 8, thermostats, the
+    electron temperature and localcolmot: 10 steps (seven redraws), the
+    observables and the final state at the tolerance. The history ring on
+    the GPU and the oracle's shifted arrays hold the same Philox elements."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(8, 300.0, 9, thermostats=True, eltemp=True, localcolmot=True)
+    p.quantumnoise, p.msteps_quantum, p.nfrequency_quantum = 1, 3, 8
+    st = consistent_state(p, at)
+    eng = Engine(p, st, compute_initial_forces=False)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=False, rng=orcpy.RNG_PHILOX)
+    oa = orc.step(0, 10)
+    ea = eng.step(0, 10)
+    for x, y in zip(oa, ea):
+        for k in ("ave_pe", "ave_ke", "tl", "te", "ts_r"):
+            assert abs(x[k] - y[k]) <= TOL * max(abs(x[k]), 1e-300), (x["step"], k, x[k], y[k])
+    _compare(eng, orc, "quantumnoise", cells=True)
+    eng.close(); orc.close()
+
+
 def test_external_field_and_force_match_oracle():
     """extfield and extforce: random per-atom fields (about 10 T) and forces
     (0.1 eV/A), 8 steps with thermostats and the electron temperature; the

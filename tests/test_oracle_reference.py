@@ -322,3 +322,60 @@ def test_live_reference_barostats():
     assert np.all(np.abs(bx[-1] - st.box) > 1e-3)
     for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def _quantum_case():
+    at = configs.make_bcc_fe(6, 300.0, 5, tilt_max_deg=10.0)
+    v = configs.fe_variables(300.0, 5, thermostats=True, eltemp=True, localcolmot=True)
+    v.update({"Msteps_quantum": 3, "Nfrequency_quantum": 8})
+    return at, v
+
+
+def _quantum_oracle(v, st, box, n):
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), box, n)
+    p.renormalize_energy = 0
+    p.quantumnoise = 1
+    return orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+
+
+@pytest.mark.skipif(not refpy.available_qn(),
+                    reason="reference with quantumnoise not built (oracle/Makefile.ref OUT=_ref/qn)")
+def test_live_reference_quantum_noise():
+    """The quantum thermostat (quantum_noise_CPU.cpp, core_dp_CPU.cpp:137-146)
+    with M = 3 and Nf = 8 on 432 atoms, eltemp and localcolmot on: the
+    16-tap filter H and 8 steps of core() (five redraws of every degree of
+    freedom's noise, each appending the MWC stream's next normal to its
+    history) bit for bit."""
+    at, v = _quantum_case()
+    st = refpy.consistent_start(v, at)
+    d = tempfile.mkdtemp()
+    ref = refpy.Reference(refpy.REF_LIB_QN)
+    ref.init_memory(d, v, st)
+    o = _quantum_oracle(v, st, st.box, st.natom)
+    np.testing.assert_array_equal(ref.quantum_H(), o.quantum_H())
+    for cur in range(8):
+        ref.run(cur, 1, False)
+        o.core(cur)
+    a = ref.atoms()
+    ref.free()
+    b = o.download()
+    o.close()
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_quantum_noise_matches_reference_fixture():
+    """quantum_fe432.npz (the reference built with -Dquantumnoise): the filter
+    and the state after 8 steps, bit for bit."""
+    g = _load("quantum_fe432.npz")
+    v = _variables(g)
+    n = g["in_r"].shape[0]
+    st = {k: g[f"in_{k}"] for k in ("r", "p", "f", "s", "stress", "rho", "ke", "pe", "me", "me0")}
+    o = _quantum_oracle(v, st, g["box"], n)
+    np.testing.assert_array_equal(o.quantum_H(), g["H"])
+    for cur in range(int(g["nsteps"])):
+        o.core(cur)
+    b = o.download()
+    o.close()
+    for k in ("r", "p", "s", "f"):
+        np.testing.assert_array_equal(b[k], g[f"final_{k}"], err_msg=k)
