@@ -1056,6 +1056,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
     TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
+    TRY(dalloc(c, &D.inc, (size_t)C * kCellCap)); TRY(dalloc(c, &D.inc_n, C));
     TRY(dalloc(c, &c->d_cl2, (size_t)C * kCellCap)); TRY(dalloc(c, &c->d_cn2, C));
     TRY(dalloc(c, &D.nb, (size_t)N * kMaxNb)); TRY(dalloc(c, &D.nbn, N));
     TRY(dalloc(c, &D.nbm, (size_t)N * kMaxMag)); TRY(dalloc(c, &D.nbmn, N));

@@ -47,6 +47,8 @@ struct Dev {
     int* cell_prev;            // link cell before the last links()
     int* cslot;                // cell * kCellCap + position in the cell's list
     int *cl, *cn;              // cell lists [C][kCellCap] head->tail, counts [C]
+    int *inc, *inc_n;          // atoms that moved into each cell this step, any order [C][kCellCap], counts [C]
+                               // (zero between steps: the relink that consumes them clears them)
     int *nb, *nbn;             // neighbours within rcut_max [N][kMaxNb]
     int *nbm, *nbmn;           // neighbours within rcut_mag [N][kMaxMag]
     double* nbJ;               // J(r) of the rcut_mag neighbours [N][kMaxMag]

@@ -97,39 +97,35 @@ __global__ __launch_bounds__(256) void k_bin(Dev D) {
     const int c = R.cell0 + lc;
     D.cell_prev[a] = old;
     D.cell[a] = c;
-    if (c != old && !cells_adjacent(R.box, old - R.cell0, lc)) atomicOr(D.err, kErrFarMove);
+    if (c != old) {
+        if (!cells_adjacent(R.box, old - R.cell0, lc)) atomicOr(D.err, kErrFarMove);
+        const int slot = atomicAdd(D.inc_n + c, 1);
+        if (slot < kCellCap) D.inc[c * kCellCap + slot] = a;
+    }
 }
 
 // Rebuild cell c's list. The reference detaches every mover and pushes it at
 // the head of its new cell in ascending atom index (links_CPU.cpp:144-173),
 // so a cell's new list is [movers in, descending index] + [stayers, old order].
+// k_bin has pushed this step's movers into the cell's incoming list in
+// arbitrary order; sorting them by index makes the list deterministic.
 __global__ __launch_bounds__(128) void k_relink(const int* cl_old, const int* cn_old, Dev D) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.C) return;
     const int rr = D.crep[c];
     if (rr < 0) return;
-    const Rep& R = D.rep[rr];
-    const int lc = c - R.cell0;
     int mv[kCellCap];
-    int nm = 0;
-    bool over = false;
-    for (int slot = 0; slot < 26; ++slot) {
-        int nc = R.cell0 + neigh_cell(R.box, lc, slot);
-        int n = cn_old[nc];
-        for (int k = 0; k < n; ++k) {
-            int a = cl_old[nc * kCellCap + k];
-            if (D.cell[a] == c) {
-                if (nm < kCellCap) {
-                    // insertion keeping descending atom index
-                    int q = nm++;
-                    while (q > 0 && mv[q - 1] < a) { mv[q] = mv[q - 1]; --q; }
-                    mv[q] = a;
-                } else {
-                    over = true;
-                }
-            }
-        }
+    const int ninc = D.inc_n[c];
+    bool over = ninc > kCellCap;
+    const int nm = over ? kCellCap : ninc;
+    for (int k = 0; k < nm; ++k) {
+        const int a = D.inc[c * kCellCap + k];
+        // insertion keeping descending atom index
+        int q = k;
+        while (q > 0 && mv[q - 1] < a) { mv[q] = mv[q - 1]; --q; }
+        mv[q] = a;
     }
+    if (ninc != 0) D.inc_n[c] = 0;
     int* out = D.cl + c * kCellCap;
     int k = 0;
     for (int q = 0; q < nm; ++q) {
@@ -478,8 +474,10 @@ __device__ __forceinline__ void exchange_field(int a, const Dev& D, const Phys& 
 //   the loads of those neighbours' spins and of atom q+1's own spin, noise
 //   and Landau coefficients, and has atom q+2's ids and density in flight.
 // The only spin such a load can miss is atom q's own new value (written after
-// the load was issued), which is patched in from registers, so every atom
-// sees exactly the spins the sequential sweep shows it.
+// the load was issued). The field is summed over the loaded spins and then
+// corrected by J_q (s_q,new - s_q,old), where J_q is atom q's weight in the
+// list (zero if it is not a neighbour), so every atom sees the spins the
+// sequential sweep shows it, up to the rounding of that correction.
 
 constexpr int kPipe = 16;   // rcut_mag neighbours held in registers (more: loaded directly)
 
@@ -508,11 +506,16 @@ struct Rot {
 };
 
 __device__ __forceinline__ Rot make_rot(double3 H, double dt) {
-    double3 w = make_double3(H.x / -kHbar, H.y / -kHbar, H.z / -kHbar);
+    constexpr double m_inv_hbar = -1.0 / kHbar;
+    double3 w = make_double3(H.x * m_inv_hbar, H.y * m_inv_hbar, H.z * m_inv_hbar);
     double w0 = sqrt(sq3(w));
     Rot r;
-    if (w0 > 0.0) { r.wx = w.x / w0; r.wy = w.y / w0; r.wz = w.z / w0; }
-    else { r.wx = 0.0; r.wy = 0.0; r.wz = 0.0; }
+    if (w0 > 0.0) {
+        const double iw = 1.0 / w0;
+        r.wx = w.x * iw; r.wy = w.y * iw; r.wz = w.z * iw;
+    } else {
+        r.wx = 0.0; r.wy = 0.0; r.wz = 0.0;
+    }
     double sn, cs;
     sincos(w0 * dt, &sn, &cs);
     r.A = sn;
@@ -612,18 +615,24 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     }
     int prev_a = -1;
     double4 prev_s = make_double4(0.0, 0.0, 0.0, 0.0);
+    double4 prev_old = prev_s;  // the previous atom's spin before its update
     for (int q = 0; q < n; ++q) {
         const int a = a_c;
-        // field of atom q (from Hext, then the gathered spins) with the
-        // previous atom's new spin patched in
-        double hx = he_c.x, hy = he_c.y, hz = he_c.z, S = 0.0;
+        // field of atom q (from Hext, then the gathered spins), then the
+        // previous atom's change of spin times its weight in the list
+        double hx = he_c.x, hy = he_c.y, hz = he_c.z, S = 0.0, Jp = 0.0;
 #pragma unroll
         for (int k = 0; k < kPipe; ++k) {
-            double4 v = id_c[k] == prev_a ? prev_s : sj[k];
-            double Jk = k < nm_c ? J_c[k] : 0.0;
+            const double4 v = sj[k];
+            const double Jk = k < nm_c ? J_c[k] : 0.0;
             hx += Jk * v.x; hy += Jk * v.y; hz += Jk * v.z;
             S += Jk * v.w;
+            Jp = (k < nm_c && id_c[k] == prev_a) ? Jk : Jp;
         }
+        hx += Jp * (prev_s.x - prev_old.x);
+        hy += Jp * (prev_s.y - prev_old.y);
+        hz += Jp * (prev_s.z - prev_old.z);
+        S += Jp * (prev_s.w - prev_old.w);
         for (int k = kPipe; k < nm_c; ++k) {
             double Jk = D.nbJ[(size_t)a * kMaxMag + k];
             double4 v = D.s[D.nbm[(size_t)a * kMaxMag + k]];
@@ -631,6 +640,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             S += Jk * v.w;
         }
         const double3 H = make_double3(hx, hy, hz);
+        prev_old = own_c;
         double3 s = d3(own_c);
         const double3 z = d3(z_c);
         const LanRows lan = lan_c;
@@ -977,14 +987,23 @@ __device__ __forceinline__ void dte_A(int c, double h, const Phys& P, const Dev&
 
 // RK4 of the electron-ion/spin relaxation over h (core_dTe_CPU.cpp:91-129);
 // G = (Gel + Ges) h / k_B
+// exp(-G / C(T)) - 1 with C(T) = 3 tanh(bT) (heatcapacity_CPU.cpp:34-57).
+// For T > 0, tanh(bT) = -v / (2 + v) with v = expm1(-2bT), so
+// -G / C = G (2 + v) / (3 v): one expm1 and one division, a third of the
+// instructions of tanh and a second division, to a few ulp.
+__device__ __forceinline__ double relax_factor(double G, double T) {
+    const double v = expm1((-2.0 * (2e-4 / kBoltz)) * T);
+    return exp(G * (2.0 + v) / (3.0 * v)) - 1.0;
+}
+
 __device__ __forceinline__ double dte_rk4(double T0, double G, double h) {
-    double k1 = T0 * (exp(-G / ce_of(T0)) - 1.0) / h;
+    double k1 = T0 * relax_factor(G, T0) / h;
     double T1 = T0 + k1 * h / 2.0;
-    double k2 = T1 * (exp(-G / ce_of(T1)) - 1.0) / h;
+    double k2 = T1 * relax_factor(G, T1) / h;
     double T2 = T0 + k2 * h / 2.0;
-    double k3 = T2 * (exp(-G / ce_of(T2)) - 1.0) / h;
+    double k3 = T2 * relax_factor(G, T2) / h;
     double T3 = T0 + k3 * h;
-    double k4 = T3 * (exp(-G / ce_of(T3)) - 1.0) / h;
+    double k4 = T3 * relax_factor(G, T3) / h;
     return T0 + h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
 }
 
@@ -1049,28 +1068,44 @@ __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, 
 }
 
 // the two totals of a sub-step: out[0] = sum of the starting-energy partials,
-// out[1] = sum of the final-energy partials; one block of 1024 threads (a
-// fixed order, independent of how the partials were produced)
+// out[1] = sum of the final-energy partials; one block of 1024 threads in a
+// fixed order, independent of how the partials were produced. Each thread
+// keeps eight loads of each array in flight, a wave sums by shuffles and
+// thread 0 adds the 16 wave sums in order.
 __global__ __launch_bounds__(1024) void k_reduce2(const double* pa, int na, const double* pb, int nb,
                                                   double* out) {
-    __shared__ double sh[2][1024];
+    __shared__ double sh[2][16];
     const int t = threadIdx.x;
     double a = 0.0, e = 0.0;
-    for (int k = t; k < na; k += 1024) a += pa[k];
-    for (int k = t; k < nb; k += 1024) e += pb[k];
-    sh[0][t] = a;
-    sh[1][t] = e;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s) {
-            sh[0][t] += sh[0][t + s];
-            sh[1][t] += sh[1][t + s];
-        }
-        __syncthreads();
+    for (int k0 = t; k0 < na; k0 += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = k0 + u * 1024 < na ? pa[k0 + u * 1024] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += v[u];
     }
+    for (int k0 = t; k0 < nb; k0 += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = k0 + u * 1024 < nb ? pb[k0 + u * 1024] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) e += v[u];
+    }
+    a = wave_sum(a);
+    e = wave_sum(e);
+    if ((t & 63) == 0) {
+        sh[0][t >> 6] = a;
+        sh[1][t >> 6] = e;
+    }
+    __syncthreads();
     if (t == 0) {
-        out[0] = sh[0][0];
-        out[1] = sh[1][0];
+        double A = 0.0, E = 0.0;
+        for (int w = 0; w < 16; ++w) {
+            A += sh[0][w];
+            E += sh[1][w];
+        }
+        out[0] = A;
+        out[1] = E;
     }
 }
 
