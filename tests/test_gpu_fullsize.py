@@ -1,0 +1,84 @@
+"""The production configuration at its full size: bcc Fe, 100^3 cubic cells =
+2,000,000 atoms, 600 K, seed 7, Langevin lattice and spin thermostats, the
+electron temperature and localcolmot, dt = 1 fs. The oracle is too slow at
+this size, so these checks are properties that hold at any size (the same
+kernels are compared with the oracle element by element at small sizes):
+- the link-cell lists partition the atoms, and after links() every atom sits
+  in the cell its position bins to (links_CPU.cpp:96-112);
+- the forces sum to zero (each pair's force is evaluated with exactly
+  negated arguments on both sides, so only the summation rounds);
+- core_dp's rescale leaves zero net momentum (core_dp_CPU.cpp:270-302);
+- the observables are the means of the per-atom fields they summarise;
+- eager launches and the per-step graph give bitwise the same trajectory."""
+import numpy as np
+import pytest
+
+from spilady_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+NU, NSTEPS = 100, 3
+
+
+@pytest.fixture(scope="module")
+def production():
+    at = configs.make_bcc_fe(NU, 600.0, 7)
+    v = configs.fe_variables(600.0, 7, thermostats=True, eltemp=True, localcolmot=True, renormalize=False)
+    return at, abi.params_from_variables(v, at.box, at.natom)
+
+
+def test_production_size_invariants(production):
+    from spilady_amd.engine import Engine
+    at, p = production
+    n = at.natom
+    e = Engine(p, at, compute_initial_forces=True)
+    obs = e.step(0, NSTEPS)
+    assert len(obs) == NSTEPS
+    a = e.download()
+    # forces: sum zero to rounding
+    f = a["f"]
+    assert np.linalg.norm(f.sum(axis=0)) <= 1e-12 * np.abs(f).sum()
+    # net momentum zero after the rescale
+    pm = a["p"]
+    assert np.linalg.norm(pm.sum(axis=0)) <= 1e-12 * np.abs(pm).sum()
+    # observables against the per-atom fields
+    o = obs[-1]
+    for key, fld in (("ave_pe", "pe"), ("ave_ke", "ke"), ("ave_me", "me")):
+        assert o[key] == pytest.approx(a[fld].mean(), rel=1e-12, abs=1e-15), key
+    np.testing.assert_allclose(o["ave_s"], a["s"].mean(axis=0), rtol=0, atol=1e-13)
+    # link cells: a partition of the atoms, consistent with the positions
+    e.links()
+    a = e.download()
+    counts, lists = e.cell_lists(32)
+    members = lists[lists >= 0]
+    assert counts.sum() == n and members.size == n
+    assert np.array_equal(np.sort(members), np.arange(n))
+    nx, ny, nz = e.ncell3
+    box = np.asarray(at.box)
+    q = a["r"] / np.array([box[0], box[2], box[5]])
+    idx = np.clip(np.trunc(q * np.array([nx, ny, nz])).astype(np.int64), 0, np.array([nx, ny, nz]) - 1)
+    cell_of = idx[:, 0] + idx[:, 1] * nx + idx[:, 2] * nx * ny
+    owner = np.repeat(np.arange(e.ncells), counts)
+    listed = np.concatenate([lists[c, :counts[c]] for c in range(e.ncells)])
+    assert np.array_equal(cell_of[listed], owner)
+    e.close()
+
+
+def test_production_size_graph_replay_is_bitwise_eager(production):
+    from spilady_amd.engine import Engine
+    at, p = production
+    out = []
+    for mode in (0, 2):
+        e = Engine(p, at, compute_initial_forces=True)
+        e.set_graph_mode(mode)
+        obs = e.step(0, 2)
+        a = e.download()
+        out.append((obs, a, e.cells()["te"]))
+        e.close()
+    (o0, a0, t0), (o2, a2, t2) = out
+    for x, y in zip(o0, o2):
+        for k in x:
+            assert np.array_equal(x[k], y[k]), k
+    for k in ("r", "p", "f", "s", "rho", "pe", "me"):
+        assert np.array_equal(a0[k], a2[k]), k
+    assert np.array_equal(t0, t2)
