@@ -225,20 +225,69 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
         int* nb = D.nb + (size_t)i * kMaxNb;
         int* nbm = D.nbm + (size_t)i * kMaxMag;
         double* nbJ = D.nbJ + (size_t)i * kMaxMag;
-        for (int q = 0; q < npass; ++q) {
-            int j;
-            double4 rj;
-            if (q < kCandReg) {
-                // registers, indexed with a compile-time constant through the switch
-                j = jc[0]; rj = rc[0];
-                if (q == 1) { j = jc[1]; rj = rc[1]; }
-                if (q == 2) { j = jc[2]; rj = rc[2]; }
-                if (q == 3) { j = jc[3]; rj = rc[3]; }
-            } else {
-                int idx = q * 64 + lane;
-                j = idx < total ? s_cand[w][idx] : -1;
-                rj = j >= 0 ? D.r[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+        // the passes held in registers: every distance first, then every
+        // table row of the pairs within rcut_max requested at once, then the
+        // list entries and density terms in pass order
+        double rsqv[kCandReg];
+        bool inmaxv[kCandReg];
+#pragma unroll
+        for (int q = 0; q < kCandReg; ++q) {
+            const bool ok = q < npass && jc[q] >= 0 && jc[q] != i;
+            double rsq = 1e300;
+            if (ok) {
+                const double3 rij = min_image(b, sub3(ri, d3(rc[q])));
+                rsq = sq3(rij);
             }
+            rsqv[q] = rsq;
+            inmaxv[q] = ok && rsq < P.rcut_max_sq;
+        }
+        double2 sfv[kCandReg], jv[kCandReg];
+        double ratv[kCandReg], rv[kCandReg];
+#pragma unroll
+        for (int q = 0; q < kCandReg; ++q) {
+            sfv[q] = make_double2(0.0, 0.0);
+            jv[q] = make_double2(0.0, 0.0);
+            ratv[q] = 0.0;
+            rv[q] = 0.0;
+            if (inmaxv[q]) {
+                const double r = sqrt(rsqv[q]);
+                int n0;
+                double ratio;
+                interp_index(r, T.in_over_rmax, n0, ratio);
+                const double* row = T.prow + (size_t)n0 * kPRowStride;
+                sfv[q] = *reinterpret_cast<const double2*>(row + P_SF);
+                jv[q] = *reinterpret_cast<const double2*>(row + P_J);
+                ratv[q] = ratio;
+                rv[q] = r;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kCandReg; ++q) {
+            if (q >= npass) break;
+            const int j = jc[q];
+            const bool inmax = inmaxv[q];
+            const unsigned long long m1 = __ballot(inmax);
+            const int pos = nn + __popcll(m1 & lt);
+            if (inmax && pos < kMaxNb) nb[pos] = j;
+            nn += __popcll(m1);
+            const bool inmag = inmax && rsqv[q] < P.rcut_mag_sq;
+            const unsigned long long m2 = __ballot(inmag);
+            const int pos2 = nm + __popcll(m2 & lt);
+            if (inmax) {
+                const double ratio = ratv[q];
+                if (rv[q] < P.rcut_pot) rho += sfv[q].x * ratio + sfv[q].y * (1.0 - ratio);
+                if (inmag && pos2 < kMaxMag) {
+                    nbm[pos2] = j;
+                    nbJ[pos2] = jv[q].x * ratio + jv[q].y * (1.0 - ratio);
+                }
+            }
+            nm += __popcll(m2);
+        }
+        // candidates past the registers (more than 64 x kCandReg): one pass at a time
+        for (int q = kCandReg; q < npass; ++q) {
+            const int idx = q * 64 + lane;
+            const int j = idx < total ? s_cand[w][idx] : -1;
+            const double4 rj = j >= 0 ? D.r[j] : make_double4(0.0, 0.0, 0.0, 0.0);
             bool ok = j >= 0 && j != i;
             double rsq = 1e300;
             if (ok) {
@@ -258,11 +307,11 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
                 int n0;
                 double ratio;
                 interp_index(r, T.in_over_rmax, n0, ratio);
-                const double* q = T.prow + (size_t)n0 * kPRowStride;
-                if (r < P.rcut_pot) rho += q[P_SF] * ratio + q[P_SF + 1] * (1.0 - ratio);
+                const double* row = T.prow + (size_t)n0 * kPRowStride;
+                if (r < P.rcut_pot) rho += row[P_SF] * ratio + row[P_SF + 1] * (1.0 - ratio);
                 if (inmag && pos2 < kMaxMag) {
                     nbm[pos2] = j;
-                    nbJ[pos2] = q[P_J] * ratio + q[P_J + 1] * (1.0 - ratio);
+                    nbJ[pos2] = row[P_J] * ratio + row[P_J + 1] * (1.0 - ratio);
                 }
             }
             nm += __popcll(m2);
@@ -302,6 +351,8 @@ __global__ __launch_bounds__(256) void k_fp(Tables T, Dev D) {
     D.fp[i] = dF + (dA * s2 + dB * s4 + dC * s4 * s2 + dD * s4 * s4);
 }
 
+constexpr int kForceBatch = 8;  // neighbour pairs whose loads the force pass issues together
+
 __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     const int lb = xcd_block((D.N + blockDim.x - 1) / blockDim.x);
     if (lb < 0) return;
@@ -321,41 +372,66 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     double st0 = 0.0, st1 = 0.0, st2 = 0.0, st3 = 0.0, st4 = 0.0, st5 = 0.0;
     const int nn = D.nbn[i];
     const int* nb = D.nb + (size_t)i * kMaxNb;
-    for (int k = 0; k < nn; ++k) {
-        int j = nb[k];
-        double3 rij = min_image(b, sub3(ri, d3(D.r[j])));
-        double rsq = sq3(rij);
-        double r = sqrt(rsq);
-        int n0;
-        double ratio;
-        interp_index(r, T.in_over_rmax, n0, ratio);
-        const double* q = T.prow + (size_t)n0 * kPRowStride;
-        const double wr = 1.0 - ratio;
-        double dudr = 0.0;
-        if (r < P.rcut_pot) {
-            double dsf = q[P_DSF] * ratio + q[P_DSF + 1] * wr;
-            double dpr = q[P_DPR] * ratio + q[P_DPR + 1] * wr;
-            double pr = q[P_PR] * ratio + q[P_PR + 1] * wr;
-            dudr = (fpi + D.fp[j]) * dsf + dpr;
-            pe += 0.5 * pr;
+    // kForceBatch pairs at a time: their neighbour ids, then their
+    // positions, F'(rho) and spins, then their table rows, all in flight
+    // together; the pair terms are then added in list order, as a
+    // pair-at-a-time loop adds them
+    for (int k0 = 0; k0 < nn; k0 += kForceBatch) {
+        int jj[kForceBatch];
+#pragma unroll
+        for (int u = 0; u < kForceBatch; ++u)
+            jj[u] = k0 + u < nn ? nb[k0 + u] : i;  // past the list: a valid slot, never added
+        double4 rj[kForceBatch], sj4[kForceBatch];
+        double fpj[kForceBatch];
+#pragma unroll
+        for (int u = 0; u < kForceBatch; ++u) {
+            rj[u] = D.r[jj[u]];
+            fpj[u] = D.fp[jj[u]];
+            sj4[u] = D.s[jj[u]];
         }
-        if (r < P.rcut_mag) {
-            double3 sj = d3(D.s[j]);
-            double sdot = dot3(si, sj);
-            double stimes = sqrt(si_sq * sq3(sj));
-            double J = q[P_J] * ratio + q[P_J + 1] * wr;
-            double dJ = q[P_DJ] * ratio + q[P_DJ + 1] * wr;
-            dudr += -dJ * (sdot - stimes);
-            double Jh = J / 2.0;
-            me0 += Jh * stimes;
-            me += -Jh * sdot;
+        double3 rij[kForceBatch];
+        double rsq[kForceBatch], r[kForceBatch], ratio[kForceBatch];
+        const double* q[kForceBatch];
+#pragma unroll
+        for (int u = 0; u < kForceBatch; ++u) {
+            rij[u] = min_image(b, sub3(ri, d3(rj[u])));
+            rsq[u] = sq3(rij[u]);
+            r[u] = sqrt(rsq[u]);
+            int n0;
+            interp_index(r[u], T.in_over_rmax, n0, ratio[u]);
+            q[u] = T.prow + (size_t)n0 * kPRowStride;
         }
-        double force = -dudr / r;
-        double3 fij = mul3(force, rij);
-        f = add3(f, fij);
-        st0 += fij.x * rij.x; st1 += fij.y * rij.y; st2 += fij.z * rij.z;
-        st3 += fij.x * rij.y; st4 += fij.y * rij.z; st5 += fij.z * rij.x;
-        vir += -force * rsq;
+#pragma unroll
+        for (int u = 0; u < kForceBatch; ++u) {
+            if (k0 + u >= nn) break;
+            const double* qq = q[u];
+            const double rt = ratio[u], wr = 1.0 - rt;
+            double dudr = 0.0;
+            if (r[u] < P.rcut_pot) {
+                double dsf = qq[P_DSF] * rt + qq[P_DSF + 1] * wr;
+                double dpr = qq[P_DPR] * rt + qq[P_DPR + 1] * wr;
+                double pr = qq[P_PR] * rt + qq[P_PR + 1] * wr;
+                dudr = (fpi + fpj[u]) * dsf + dpr;
+                pe += 0.5 * pr;
+            }
+            if (r[u] < P.rcut_mag) {
+                double3 sj = d3(sj4[u]);
+                double sdot = dot3(si, sj);
+                double stimes = sqrt(si_sq * sq3(sj));
+                double J = qq[P_J] * rt + qq[P_J + 1] * wr;
+                double dJ = qq[P_DJ] * rt + qq[P_DJ + 1] * wr;
+                dudr += -dJ * (sdot - stimes);
+                double Jh = J / 2.0;
+                me0 += Jh * stimes;
+                me += -Jh * sdot;
+            }
+            double force = -dudr / r[u];
+            double3 fij = mul3(force, rij[u]);
+            f = add3(f, fij);
+            st0 += fij.x * rij[u].x; st1 += fij.y * rij[u].y; st2 += fij.z * rij[u].z;
+            st3 += fij.x * rij[u].y; st4 += fij.y * rij[u].z; st5 += fij.z * rij[u].x;
+            vir += -force * rsq[u];
+        }
     }
     // one-body terms (calculate_force_energy_CPU.cpp:196-215)
     int n0;
