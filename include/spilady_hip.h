@@ -224,6 +224,8 @@ int sld_download(sld_ctx* ctx, sld_atoms* out);
  * the cell lists keep their current order. */
 int sld_upload(sld_ctx* ctx, const sld_atoms* in);
 int sld_get_cells(sld_ctx* ctx, sld_cells* out);
+/* Te (k_B T, eV) of every cell of the selected replica, in cell order, and
+ * Ee = E(Te) with it: the readTe start (map_cells.cpp:136-157). */
 int sld_set_cells_te(sld_ctx* ctx, const double* te);
 /* Link-cell geometry and groups (map_cells.cpp:41-60, allocate_cells_CPU.cpp). */
 int sld_get_geometry(sld_ctx* ctx, int* ncell3, int* ngroups, int* group_sizes);

@@ -1537,7 +1537,11 @@ void orc_get_cells(const orc_ctx* o, sld_cells* out) {
     store1(out->sum_l_dn, o->sum_L_dn, C);
 }
 
-void orc_set_cells_te(orc_ctx* o, const double* te) { memcpy(o->Te, te, sizeof(double) * o->ncells); }
+/* Te of every cell and Ee = E(Te), as the readTe start sets them (map_cells.cpp:136-157) */
+void orc_set_cells_te(orc_ctx* o, const double* te) {
+    memcpy(o->Te, te, sizeof(double) * o->ncells);
+    for (int c = 0; c < o->ncells; ++c) o->Ee[c] = Te_to_Ee(o->Te[c]);
+}
 
 void orc_get_geometry(const orc_ctx* o, int* ncell3, int* ngroups, int* group_sizes) {
     if (ncell3) { ncell3[0] = o->nx; ncell3[1] = o->ny; ncell3[2] = o->nz; }

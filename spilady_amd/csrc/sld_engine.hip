@@ -1407,8 +1407,11 @@ int sld_get_cells(sld_ctx* c, sld_cells* o) {
 }
 
 int sld_set_cells_te(sld_ctx* c, const double* te) {
+    if (!c || !te) return SLD_ERR_ARG;
     const Rep& R = c->reps[c->sel];
     TRY(up1(c, c->D.Te, te, R.cell0, R.ncell));
+    LAUNCH(c, k_cells_ee, nblk(R.ncell, 256), 256, c->D, R.cell0, R.ncell);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SLD_OK;
 }
 

@@ -1392,6 +1392,13 @@ __global__ __launch_bounds__(256) void k_renorm(Dev D, const double* err, double
     block_partials<1>(acc, partials);
 }
 
+// Ee = E(Te) over the cells [c0, c0 + n) (map_cells.cpp:150-157)
+__global__ __launch_bounds__(256) void k_cells_ee(Dev D, int c0, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    D.Ee[c0 + k] = te_to_ee(D.Te[c0 + k]);
+}
+
 __global__ __launch_bounds__(256) void k_check_nonfinite(Dev D) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= D.N) return;

@@ -58,7 +58,9 @@ struct Vars {
         {"stress_yx", "0"}, {"stress_yy", "0"}, {"stress_zx", "0"}, {"stress_zy", "0"}, {"stress_zz", "0"},
         {"interval_of_scale_pressure", "1"}, {"pressure", "0"}, {"baro_damping_time", "1e-13"},
         // quantum thermostat: control.h quantumnoise, default.h:222-225
-        {"quantumnoise", "0"}, {"Msteps_quantum", "50"}, {"Nfrequency_quantum", "150"}};
+        {"quantumnoise", "0"}, {"Msteps_quantum", "50"}, {"Nfrequency_quantum", "150"},
+        // control.h readTe: the cells' electron temperatures from a cel- file (default.h:83-85)
+        {"readTe", "0"}, {"in_eltemp", "cel-input.dat"}};
     void read(const char* path) {
         std::ifstream in(path);
         if (!in) fail(std::string("cannot open ") + path);
@@ -306,6 +308,36 @@ int main(int argc, char** argv) {
     }
 
     if (barostats && sld_set_barostat(ctx, &baro)) fail(sld_last_error(ctx));
+
+    // readTe (map_cells.cpp:33-60,125-160): "ncells total_time", two box lines,
+    // then per cell, x outermost and z innermost, "x y z Te(K) Tl Ts_R Ts_L";
+    // the cells take Te and Ee = E(Te). The file must describe this box.
+    if (prm.eltemp && v.i("readTe")) {
+        const std::string name = v.s("in_eltemp");
+        std::ifstream in(name);
+        if (!in) fail("cannot open " + name);
+        int n3[3];
+        if (sld_get_geometry(ctx, n3, nullptr, nullptr)) fail(sld_last_error(ctx));
+        const int C = n3[0] * n3[1] * n3[2];
+        int nc = 0;
+        double t = 0.0, bx[6];
+        if (!(in >> nc >> t)) fail("cannot read " + name);
+        for (double& x : bx)
+            if (!(in >> x)) fail("cannot read the box of " + name);
+        if (nc != C) fail(name + ": the number of link cells differs from the configuration's");
+        for (int k = 0; k < 6; ++k)
+            if (std::fabs(bx[k] - prm.box[k]) > 1e-9 * (1.0 + std::fabs(prm.box[k])))
+                fail(name + ": the box differs from the configuration's");
+        std::vector<double> te(C);
+        for (int ix = 0; ix < n3[0]; ++ix)
+            for (int iy = 0; iy < n3[1]; ++iy)
+                for (int iz = 0; iz < n3[2]; ++iz) {
+                    double x, y, z, T, tl, tr, ts;
+                    if (!(in >> x >> y >> z >> T >> tl >> tr >> ts)) fail("cannot read the cells of " + name);
+                    te[ix + iy * n3[0] + iz * n3[0] * n3[1]] = T * kBoltz;
+                }
+        if (sld_set_cells_te(ctx, te.data())) fail(sld_last_error(ctx));
+    }
 
     // check(-1) and write(-1) (spilady.cpp:38-40)
     sld_obs o;

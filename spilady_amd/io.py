@@ -71,6 +71,9 @@ VARIABLE_DEFAULTS: Dict[str, object] = {
     "quantumnoise": 0,
     "Msteps_quantum": 50,
     "Nfrequency_quantum": 150,
+    # control.h readTe: cell electron temperatures from a cel- file (default.h:83-85)
+    "readTe": 0,
+    "in_eltemp": "cel-input.dat",
 }
 
 
@@ -221,6 +224,18 @@ def write_cel(path: str, box, ncell3, te_ev, tl, ts_r, ts_l, total_time: float) 
                              f"{tl[c]:.15e} {ts_r[c]:.15e} {ts_l[c]:.15e} ")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
+
+
+def read_cel(path: str):
+    """cel- file as the readTe start reads it (map_cells.cpp:33-60,125-160):
+    returns (total_time, box[6], columns[C][7]) with the cells in file order
+    (x outermost, z innermost) and Te in K."""
+    with open(path) as fh:
+        toks = fh.read().split()
+    nc, t = int(toks[0]), float(toks[1])
+    box = np.array([float(x) for x in toks[2:8]])
+    cols = np.array([float(x) for x in toks[8:8 + 7 * nc]]).reshape(nc, 7)
+    return t, box, cols
 
 
 def read_log(path: str) -> np.ndarray:

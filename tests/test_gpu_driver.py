@@ -193,3 +193,45 @@ def test_driver_barostats_log_the_moving_box(tmp_path):
         np.testing.assert_array_equal(getattr(fin, k), np.vectorize(fmt)(a[k]), err_msg=k)
     b = final_box
     assert np.all(fin.local_volume == fmt(b[0] * b[2] * b[5] / st.natom))
+
+
+def test_driver_reads_cell_temperatures(tmp_path):
+    """readTe: the cells start from a cel- file's electron temperatures (K)
+    with Ee = E(Te); the driver's logs equal the in-process engine's given
+    the same Te through sld_set_cells_te."""
+    from spilady_amd import abi
+    from spilady_amd import constants as K
+    from spilady_amd.engine import Engine
+    tmp = str(tmp_path)
+    v = _setup(tmp, True, readTe=1, in_eltemp="cel-start.dat", interval_of_print_out=1)
+    st = io.read_con(os.path.join(tmp, "con-start.dat"))
+    p = abi.params_from_variables(io.read_variables(os.path.join(tmp, "variables.in")), st.box, st.natom)
+    e = Engine(p, st, compute_initial_forces=False)
+    C = e.ncells
+    rng = np.random.default_rng(12)
+    te = rng.uniform(300.0, 1500.0, C) * K.BOLTZ
+    zero = np.zeros(C)
+    io.write_cel(os.path.join(tmp, "cel-start.dat"), st.box, e.ncell3, te, zero, zero, zero, 0.0)
+    # what the driver reads back: Te in K at 15 digits
+    _, _, cols = io.read_cel(os.path.join(tmp, "cel-start.dat"))
+    nx, ny, nz = e.ncell3
+    te_read = np.zeros(C)
+    k = 0
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
+                te_read[ix + iy * nx + iz * nx * ny] = cols[k, 3] * K.BOLTZ
+                k += 1
+    e.set_cells_te(te_read)
+    obs = [e.check(-1)] + e.step(0, 7)
+    e.close()
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    fmt = lambda x: float("%.15e" % x)  # noqa: E731
+    enr = io.read_log(os.path.join(tmp, "enr-drv.dat"))
+    tmpl = io.read_log(os.path.join(tmp, "tmp-drv.dat"))
+    assert enr.shape[0] == len(obs) == 8
+    assert obs[0]["te"] == pytest.approx(te_read.mean() / K.BOLTZ, rel=1e-12)
+    for row_e, row_t, o in zip(enr, tmpl, obs):
+        assert list(row_e[2:]) == [fmt(x) for x in (o["ave_pe"], o["ave_ke"], o["ave_me"], o["ave_me0"], o["ave_ee"])]
+        assert list(row_t[2:]) == [fmt(x) for x in (o["tl"], o["ts_r"], o["ts_l"], o["te"])]

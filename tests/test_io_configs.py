@@ -85,3 +85,25 @@ def test_driver_reports_missing_inputs(tmp_path):
     (tmp_path / "variables.in").write_text("in_config con-missing.dat\n")
     r = subprocess.run([drv], cwd=tmp_path, capture_output=True, text=True, timeout=30)
     assert r.returncode == 1 and "con-missing.dat" in r.stderr
+
+
+def test_cel_file_round_trip(tmp_path):
+    """cel- files as write_config.cpp:160-200 writes them and the readTe start
+    reads them (map_cells.cpp:33-60,125-160): Te in K, cells x outermost."""
+    rng = np.random.default_rng(3)
+    n3 = (4, 3, 5)
+    C = int(np.prod(n3))
+    box = [12.0, 0.0, 9.0, 0.0, 0.0, 15.0]
+    te = rng.uniform(300.0, 900.0, C) * K.BOLTZ
+    tl, tr, ts = (rng.uniform(200.0, 800.0, C) for _ in range(3))
+    path = str(tmp_path / "cel-x_0001.dat")
+    io.write_cel(path, box, n3, te, tl, tr, ts, 2.5e-13)
+    t, b, cols = io.read_cel(path)
+    assert t == 2.5e-13 and np.array_equal(b, box) and cols.shape == (C, 7)
+    order = [ix + iy * n3[0] + iz * n3[0] * n3[1] for ix in range(n3[0]) for iy in range(n3[1]) for iz in range(n3[2])]
+    np.testing.assert_allclose(cols[:, 3] * K.BOLTZ, te[order], rtol=1e-14)
+    np.testing.assert_array_equal(cols[:, 4], [float("%.15e" % x) for x in tl[order]])
+    # cell centres from the box matrix
+    ix, iy, iz = order[7] % n3[0], (order[7] // n3[0]) % n3[1], order[7] // (n3[0] * n3[1])
+    np.testing.assert_allclose(cols[7, :3], [(ix + 0.5) / n3[0] * 12.0, (iy + 0.5) / n3[1] * 9.0,
+                                             (iz + 0.5) / n3[2] * 15.0], rtol=1e-14)
