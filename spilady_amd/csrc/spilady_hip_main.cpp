@@ -44,7 +44,7 @@ constexpr double kMuB = 5.7883817555e-5;     // eV/T
 // defaults are the reference's (default.h:44-226)
 struct Vars {
     std::map<std::string, std::string> kv = {
-        {"out_body", "spilady"}, {"no_of_production_steps", "1000"}, {"step", "1e-15"},
+        {"OMP_threads", "2"}, {"out_body", "spilady"}, {"no_of_production_steps", "1000"}, {"step", "1e-15"},
         {"interval_of_print_out", "1"}, {"interval_of_config_out", "1000"}, {"in_config", "con-input.dat"},
         {"kappa_e", "80"}, {"start_time", "0"}, {"atmass", "55.847"}, {"temperature", "300"},
         {"gamma_L_over_mass", "6e11"}, {"gamma_S_HL", "5.88e13"}, {"random_seed", "1234"},
@@ -222,6 +222,53 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
 
 }  // namespace
 
+// variables.all (read_variables.cpp:278-418): the variables in effect, in
+// the reference's order and with the stream's default formatting, for the
+// switches this driver runs. OMP_threads is echoed as the reference's OpenMP
+// build does; the V_sim keys are left out, since no V_sim files are written.
+void write_variables_all(const Vars& v, const sld_params& prm, const sld_barostat& baro, bool runstep,
+                         bool read_te) {
+    std::ofstream out("variables.all");
+    if (!out) fail("cannot write variables.all");
+    out << "OMP_threads " << v.i("OMP_threads") << '\n';
+    out << "out_body " << v.s("out_body") << '\n';
+    if (runstep) out << "no_of_production_steps " << v.i("no_of_production_steps") << '\n';
+    else out << "total_production_time " << v.d("total_production_time") << '\n';
+    out << "step " << prm.step << '\n';
+    out << "interval_of_print_out " << prm.interval_of_print_out << '\n';
+    out << "interval_of_config_out " << v.i("interval_of_config_out") << '\n';
+    out << "in_config " << v.s("in_config") << '\n';
+    if (read_te) out << "in_eltemp " << v.s("in_eltemp") << '\n';
+    if (prm.eltemp) out << "kappa_e " << prm.kappa_e << '\n';
+    out << "atmass " << prm.atmass / (kUtoEv * std::pow(kLengthScale, 2)) << '\n';
+    if (prm.lattlang || prm.spinlang) out << "temperature " << prm.temperature / kBoltz << '\n';
+    if (prm.lattlang) out << "gamma_L_over_mass " << prm.gamma_L_over_mass << '\n';
+    if (prm.spinlang) out << "gamma_S_HL " << prm.gamma_S_HL << '\n';
+    if (baro.stress_on) {
+        out << "interval_of_scale_stress " << baro.interval_of_scale_stress << '\n';
+        const char* names[6] = {"stress_xx", "stress_yx", "stress_yy", "stress_zx", "stress_zy", "stress_zz"};
+        for (int k = 0; k < 6; ++k) out << names[k] << " " << baro.stress[k] << '\n';
+    }
+    if (baro.pressure_on) {
+        out << "interval_of_scale_pressure " << baro.interval_of_scale_pressure << '\n';
+        out << "pressure " << baro.pressure << '\n';
+    }
+    if (baro.stress_on || baro.pressure_on) out << "baro_damping_time " << baro.baro_damping_time << '\n';
+    out << "random_seed " << prm.random_seed << '\n';
+    out << "rcut_pot " << prm.rcut_pot << '\n';
+    out << "rcut_mag " << prm.rcut_mag << '\n';
+    out << "rcut_max " << prm.rcut_max << '\n';
+    out << "min_length_link_cell " << prm.min_length_link_cell << '\n';
+    if (prm.changestep) {
+        out << "displace_limit " << prm.displace_limit << '\n';
+        out << "phi_limit " << prm.phi_limit << '\n';
+    }
+    if (prm.quantumnoise) {
+        out << "Msteps_quantum " << prm.msteps_quantum << '\n';
+        out << "Nfrequency_quantum " << prm.nfrequency_quantum << '\n';
+    }
+}
+
 int main(int argc, char** argv) {
     Vars v;
     v.read(argc > 1 ? argv[1] : "variables.in");
@@ -269,6 +316,7 @@ int main(int argc, char** argv) {
     baro.baro_damping_time = v.d("baro_damping_time");
     const bool barostats = baro.stress_on || baro.pressure_on;
     const bool runstep = v.i("runstep") != 0;
+    write_variables_all(v, prm, baro, runstep, prm.eltemp && v.i("readTe"));
     const double total_production_time = v.d("total_production_time");
     const std::string body = v.s("out_body");
     const long long nsteps = v.i("no_of_production_steps");

@@ -235,3 +235,73 @@ def test_driver_reads_cell_temperatures(tmp_path):
     for row_e, row_t, o in zip(enr, tmpl, obs):
         assert list(row_e[2:]) == [fmt(x) for x in (o["ave_pe"], o["ave_ke"], o["ave_me"], o["ave_me0"], o["ave_ee"])]
         assert list(row_t[2:]) == [fmt(x) for x in (o["tl"], o["ts_r"], o["ts_l"], o["te"])]
+
+
+def test_driver_variables_all_matches_reference(tmp_path):
+    """variables.all (read_variables.cpp:278-418): for tests/golden/variables_in.txt
+    (the shipped control.h's switches) the driver's echo equals the one the
+    compiled reference wrote, line for line, except the two V_sim keys (this
+    build writes no V_sim files)."""
+    from spilady_amd import abi
+    tmp = str(tmp_path)
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(gold, "variables_in.txt")) as fh, open(os.path.join(tmp, "variables.in"), "w") as out:
+        out.write(fh.read())
+    v = io.read_variables(os.path.join(tmp, "variables.in"))
+    at = configs.make_bcc_fe(8, 600.0, 21)
+    st = consistent_state(abi.params_from_variables(v, at.box, at.natom), at)
+    state = io.AtomState(box=np.asarray(at.box), r=st["r"], p=st["p"], f=st["f"], s=st["s"],
+                         stress=st["stress"], rho=st["rho"], ke=st["ke"], pe=st["pe"], me=st["me"],
+                         me0=st["me0"])
+    io.write_con(os.path.join(tmp, v["in_config"]), state)
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    with open(os.path.join(tmp, "variables.all")) as fh:
+        ours = fh.read().splitlines()
+    with open(os.path.join(gold, "variables_all_ref.txt")) as fh:
+        ref = [ln for ln in fh.read().splitlines() if ln.split()[0] not in ("interval_of_vsim", "vsim_prec")]
+    assert ours == ref
+
+
+def test_driver_checkpoint_resume(tmp_path):
+    """Checkpoint and resume through the con- files (write_config.cpp:31-131,
+    read_config.cpp:32-147): 6 NVE steps in one run against 3 steps, then 3
+    more started from the con- file written after step 3 with start_time set
+    to its time (inittime). The con- file holds 16 significant digits, so the
+    resumed run agrees to 1e-10, not bitwise: positions, momenta, spins and
+    the energies of the last three printed steps."""
+    from helpers import fe_params
+    tmp = str(tmp_path)
+    v, p, at = fe_params(8, 600.0, 23, thermostats=False)
+    st = consistent_state(p, at)
+    state = io.AtomState(box=np.asarray(at.box), r=st["r"], p=st["p"], f=st["f"], s=st["s"],
+                         stress=st["stress"], rho=st["rho"], ke=st["ke"], pe=st["pe"], me=st["me"],
+                         me0=st["me0"])
+    full, part = os.path.join(tmp, "full"), os.path.join(tmp, "part")
+    for d in (full, part):
+        os.makedirs(d)
+        io.write_con(os.path.join(d, "con-start.dat"), state)
+    base = dict(v, in_config="con-start.dat", out_body="drv", interval_of_print_out=1, interval_of_config_out=3)
+    io.write_variables(os.path.join(full, "variables.in"), dict(base, no_of_production_steps=6))
+    io.write_variables(os.path.join(part, "variables.in"), dict(base, no_of_production_steps=3))
+    for d in (full, part):
+        res = subprocess.run([DRIVER, "variables.in"], cwd=d, capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+    ck = io.read_con(os.path.join(part, "con-drv_0001.dat"))
+    t3 = float(io.read_log(os.path.join(part, "enr-drv.dat"))[-1, 1])
+    resume = os.path.join(tmp, "resume")
+    os.makedirs(resume)
+    io.write_con(os.path.join(resume, "con-ck.dat"), ck)
+    io.write_variables(os.path.join(resume, "variables.in"),
+                       dict(base, in_config="con-ck.dat", no_of_production_steps=3, start_time=t3))
+    res = subprocess.run([DRIVER, "variables.in"], cwd=resume, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    a = io.read_con(os.path.join(full, "con-drv_9999.dat"))
+    b = io.read_con(os.path.join(resume, "con-drv_9999.dat"))
+    for k in ("r", "p", "s"):
+        x, y = getattr(a, k), getattr(b, k)
+        assert np.linalg.norm(x - y) <= 1e-10 * np.linalg.norm(x), k
+    ea = io.read_log(os.path.join(full, "enr-drv.dat"))[-3:]
+    eb = io.read_log(os.path.join(resume, "enr-drv.dat"))[-3:]
+    np.testing.assert_allclose(eb[:, 1], ea[:, 1], rtol=1e-12)           # total time continues
+    np.testing.assert_allclose(eb[:, 2:], ea[:, 2:], rtol=1e-10, atol=1e-12)
