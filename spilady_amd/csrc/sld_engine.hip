@@ -446,7 +446,9 @@ int do_dTe_launches(sld_ctx* c, double dt) {
     double vf = atomic_volume / (natom / ncell);
     double pref = dtl / 2.0 * kappa * vf / kBoltz;
     int nb = nblk(c->D.C, 256);
-    LAUNCH(c, k_dte_ab, nb, 256, h, c->phys, c->D, c->d_part);
+    const Mesh M{c->D.Te, c->D.Ee, c->D.Gel, c->D.Ges, c->D.Tl, c->D.TsL, c->D.C, c->phys.lattlang,
+                 c->phys.spinlang};
+    LAUNCH(c, k_dte_ab, nb, 256, h, M, c->d_part);
     // block partials: [0, nb) starting energies (k_dte_ab / k_dte_finish_ab),
     // then the final energies of each backward level in turn
     for (int it = 0; it < nloop; ++it) {
@@ -464,8 +466,9 @@ int do_dTe_launches(sld_ctx* c, double dt) {
                 int n = L.off[l + 1] - L.off[l];
                 double* part = dir == 1 ? c->d_part + (size_t)nb + npb : nullptr;
                 const int bs = c->shapes.sweep_block;
+                const Box& bx = c->reps[0].box;
                 LAUNCH(c, k_dte_sweep, nblk(n, bs), bs, L.d_cells + L.off[l], n, pref, h0.dx2,
-                       h0.dy2, h0.dz2, c->reps[0].box, c->D, part);
+                       h0.dy2, h0.dz2, bx.nx, bx.ny, bx.nz, c->D.Te, c->D.Ee, part);
                 if (dir == 1) npb += nblk(n, 64);  // one partial per wave
                 nlaunch += n > 0;
             }
@@ -478,9 +481,9 @@ int do_dTe_launches(sld_ctx* c, double dt) {
         LAUNCH(c, k_reduce2, 1, 1024, (const double*)c->d_part, nb, (const double*)(c->d_part + nb), npb,
                c->d_scal + 8);
         if (it + 1 < nloop)
-            LAUNCH(c, k_dte_finish_ab, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8), 1, c->d_part);
+            LAUNCH(c, k_dte_finish_ab, nb, 256, h, M, (const double*)(c->d_scal + 8), 1, c->d_part);
         else
-            LAUNCH(c, k_dte_finish, nb, 256, h, c->phys, c->D, (const double*)(c->d_scal + 8), 1);
+            LAUNCH(c, k_dte_finish, nb, 256, h, M, (const double*)(c->d_scal + 8), 1);
     }
     return SLD_OK;
 }

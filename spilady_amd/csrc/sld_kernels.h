@@ -58,6 +58,14 @@ struct Dev {
     int* err;                  // error flag bits
 };
 
+// The electron mesh as the core_dTe sub-step kernels read it: only what they
+// use, since a 2M-atom step launches about 100 of them and 3,600 sweeps.
+struct Mesh {
+    double *Te, *Ee;
+    const double *Gel, *Ges, *Tl, *TsL;
+    int C, lattlang, spinlang;
+};
+
 // Parameters shared by every replica of a batch.
 struct Phys {
     double atmass, gamma_L, gamma_S;
@@ -91,12 +99,14 @@ __global__ void k_dp_center(Phys P, Dev D, const double* scal, double* partials)
 __global__ void k_dp_scale(Phys P, Dev D, const double* scal);
 __global__ void k_dr(double dt, Phys P, Dev D);
 __global__ void k_temperature(Tables T, Phys P, Dev D);
-__global__ void k_dte_ab(double h, Phys P, Dev D, double* partials);
+__global__ void k_dte_ab(double h, Mesh M, double* partials);
+// the arguments are the few the stencil reads: one of these launches per
+// level and sub-step, 3,600 per 2M-atom step
 __global__ void k_dte_sweep(const int* cells, int ncell, double pref, double dx2, double dy2,
-                            double dz2, Box b, Dev D, double* partials);
+                            double dz2, int nx, int ny, int nz, double* Te, double* Ee, double* partials);
 __global__ void k_reduce2(const double* pa, int na, const double* pb, int nb, double* out);
-__global__ void k_dte_finish(double h, Phys P, Dev D, const double* xp, int nxp);
-__global__ void k_dte_finish_ab(double h, Phys P, Dev D, const double* xp, int nxp, double* partials);
+__global__ void k_dte_finish(double h, Mesh M, const double* xp, int nxp);
+__global__ void k_dte_finish_ab(double h, Mesh M, const double* xp, int nxp, double* partials);
 __global__ void k_obs_atoms(Tables T, Phys P, Dev D, double* partials, int scan_spin);
 __global__ void k_obs_cells(Dev D, double* partials);
 __global__ void k_renorm(Dev D, const double* err, double* partials);

@@ -1051,14 +1051,18 @@ __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
 
 // core_dTe sub-step, first half: A(h), B(h), then Ee = E(Te) as C starts
 // (core_dTe_CPU.cpp:51-55,66-129,133-140)
-__device__ __forceinline__ void dte_A(int c, double h, const Phys& P, const Dev& D) {
-    double ee = te_to_ee(D.Te[c]);
+__device__ __forceinline__ double dte_dE(const Mesh& M, int c) {
     double dE = 0.0;
-    if (P.lattlang) dE += D.Gel[c] * D.Tl[c];
-    if (P.spinlang) dE += D.Ges[c] * D.TsL[c];
-    ee += h * dE;
-    D.Ee[c] = ee;
-    D.Te[c] = ee_to_te(ee);
+    if (M.lattlang) dE += M.Gel[c] * M.Tl[c];
+    if (M.spinlang) dE += M.Ges[c] * M.TsL[c];
+    return dE;
+}
+
+__device__ __forceinline__ void dte_A(int c, double h, const Mesh& M) {
+    double ee = te_to_ee(M.Te[c]);
+    ee += h * dte_dE(M, c);
+    M.Ee[c] = ee;
+    M.Te[c] = ee_to_te(ee);
 }
 
 // RK4 of the electron-ion/spin relaxation over h (core_dTe_CPU.cpp:91-129);
@@ -1083,25 +1087,25 @@ __device__ __forceinline__ double dte_rk4(double T0, double G, double h) {
     return T0 + h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
 }
 
-__device__ __forceinline__ double dte_G(int c, double h, const Phys& P, const Dev& D) {
+__device__ __forceinline__ double dte_G(int c, double h, const Mesh& M) {
     double G = 0.0;
-    if (P.lattlang) G += D.Gel[c];
-    if (P.spinlang) G += D.Ges[c];
+    if (M.lattlang) G += M.Gel[c];
+    if (M.spinlang) G += M.Ges[c];
     return G * (h / kBoltz);
 }
 
-__device__ __forceinline__ void dte_B(int c, double h, const Phys& P, const Dev& D) {
-    D.Te[c] = dte_rk4(D.Te[c], dte_G(c, h, P, D), h);
+__device__ __forceinline__ void dte_B(int c, double h, const Mesh& M) {
+    M.Te[c] = dte_rk4(M.Te[c], dte_G(c, h, M), h);
 }
 
-__global__ __launch_bounds__(256) void k_dte_ab(double h, Phys P, Dev D, double* partials) {
+__global__ __launch_bounds__(256) void k_dte_ab(double h, Mesh M, double* partials) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
-    if (c < D.C) {
-        dte_A(c, h, P, D);
-        dte_B(c, h, P, D);
-        double ee = te_to_ee(D.Te[c]);
-        D.Ee[c] = ee;
+    if (c < M.C) {
+        dte_A(c, h, M);
+        dte_B(c, h, M);
+        double ee = te_to_ee(M.Te[c]);
+        M.Ee[c] = ee;
         acc[0] = ee;
     }
     block_partials<1>(acc, partials);
@@ -1115,26 +1119,26 @@ __global__ __launch_bounds__(256) void k_dte_ab(double h, Phys P, Dev D, double*
 // launch also leaves partials of the final energies (partials != null), one
 // per wave of 64 level entries: the sums do not depend on the block size.
 __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref, double dx2,
-                                                   double dy2, double dz2, Box b, Dev D, double* partials) {
+                                                   double dy2, double dz2, int nx, int ny, int nz, double* Te,
+                                                   double* Ee, double* partials) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (t < ncell) {
         const int c = cells[t];
-        const int nx = b.nx, ny = b.ny, nz = b.nz, sl = nx * ny;
+        const int sl = nx * ny;
         const int ix = c % nx, iy = (c / nx) % ny, iz = c / sl;
         const int xm = ix == 0 ? nx - 1 : ix - 1, xp = ix + 1 == nx ? 0 : ix + 1;
         const int ym = iy == 0 ? ny - 1 : iy - 1, yp = iy + 1 == ny ? 0 : iy + 1;
         const int zm = iz == 0 ? nz - 1 : iz - 1, zp = iz + 1 == nz ? 0 : iz + 1;
         const int row = iy * nx + iz * sl;
-        const double* Te = D.Te;
         const double tc = Te[c];
         const double txm = Te[xm + row], txp = Te[xp + row];
         const double tym = Te[ix + ym * nx + iz * sl], typ = Te[ix + yp * nx + iz * sl];
         const double tzm = Te[ix + iy * nx + zm * sl], tzp = Te[ix + iy * nx + zp * sl];
-        double ee = D.Ee[c];
+        double ee = Ee[c];
         ee += pref * ((txp + txm - 2.0 * tc) / dx2 + (typ + tym - 2.0 * tc) / dy2 + (tzp + tzm - 2.0 * tc) / dz2);
-        D.Ee[c] = ee;
-        D.Te[c] = ee_to_te(ee);
+        Ee[c] = ee;
+        Te[c] = ee_to_te(ee);
         acc[0] = ee;
     }
     if (partials) {
@@ -1201,39 +1205,37 @@ __device__ __forceinline__ double dte_factor(const double* xp, int n) {
 // In registers: the Te and Ee that the first A(h) leaves are overwritten by
 // the second before anything reads them, so only the final values are
 // computed and stored (same arithmetic as B, A, A, B applied in turn).
-__global__ __launch_bounds__(256) void k_dte_finish_ab(double h, Phys P, Dev D, const double* xp, int nxp,
+__global__ __launch_bounds__(256) void k_dte_finish_ab(double h, Mesh M, const double* xp, int nxp,
                                                        double* partials) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
-    if (c < D.C) {
+    if (c < M.C) {
         const double f = dte_factor(xp, nxp);
-        const double G = dte_G(c, h, P, D);
-        double dE = 0.0;
-        if (P.lattlang) dE += D.Gel[c] * D.Tl[c];
-        if (P.spinlang) dE += D.Ges[c] * D.TsL[c];
-        double T = ee_to_te(D.Ee[c] * f);
+        const double G = dte_G(c, h, M);
+        const double dE = dte_dE(M, c);
+        double T = ee_to_te(M.Ee[c] * f);
         T = dte_rk4(T, G, h);                  // B(h) ends the sub-step
         double ee = te_to_ee(T) + h * dE;      // A(h) ends it
         ee += h * dE;                          // A(h) starts the next
         T = dte_rk4(ee_to_te(ee), G, h);       // B(h)
         ee = te_to_ee(T);                      // E(Te) as C starts
-        D.Te[c] = T;
-        D.Ee[c] = ee;
+        M.Te[c] = T;
+        M.Ee[c] = ee;
         acc[0] = ee;
     }
     block_partials<1>(acc, partials);
 }
 
 // renormalise to the starting total (core_dTe_CPU.cpp:199-216), then B(h), A(h)
-__global__ __launch_bounds__(256) void k_dte_finish(double h, Phys P, Dev D, const double* xp, int nxp) {
+__global__ __launch_bounds__(256) void k_dte_finish(double h, Mesh M, const double* xp, int nxp) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= D.C) return;
+    if (c >= M.C) return;
     double f = dte_factor(xp, nxp);
-    double ee = D.Ee[c] * f;
-    D.Ee[c] = ee;
-    D.Te[c] = ee_to_te(ee);
-    dte_B(c, h, P, D);
-    dte_A(c, h, P, D);
+    double ee = M.Ee[c] * f;
+    M.Ee[c] = ee;
+    M.Te[c] = ee_to_te(ee);
+    dte_B(c, h, M);
+    dte_A(c, h, M);
 }
 
 
