@@ -1065,8 +1065,6 @@ __device__ __forceinline__ void dte_A(int c, double h, const Mesh& M) {
     M.Te[c] = ee_to_te(ee);
 }
 
-// RK4 of the electron-ion/spin relaxation over h (core_dTe_CPU.cpp:91-129);
-// G = (Gel + Ges) h / k_B
 // exp(-G / C(T)) - 1 with C(T) = 3 tanh(bT) (heatcapacity_CPU.cpp:34-57).
 // For T > 0, tanh(bT) = -v / (2 + v) with v = expm1(-2bT), so
 // -G / C = G (2 + v) / (3 v): one expm1 and one division, a third of the
@@ -1076,15 +1074,19 @@ __device__ __forceinline__ double relax_factor(double G, double T) {
     return exp(G * (2.0 + v) / (3.0 * v)) - 1.0;
 }
 
-__device__ __forceinline__ double dte_rk4(double T0, double G, double h) {
-    double k1 = T0 * relax_factor(G, T0) / h;
-    double T1 = T0 + k1 * h / 2.0;
-    double k2 = T1 * relax_factor(G, T1) / h;
-    double T2 = T0 + k2 * h / 2.0;
-    double k3 = T2 * relax_factor(G, T2) / h;
-    double T3 = T0 + k3 * h;
-    double k4 = T3 * relax_factor(G, T3) / h;
-    return T0 + h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+// RK4 of the electron-ion/spin relaxation over h (core_dTe_CPU.cpp:91-129),
+// G = (Gel + Ges) h / k_B. The stages are carried as k h (the reference
+// divides each k by h and multiplies it back): the same increments up to
+// rounding.
+__device__ __forceinline__ double dte_rk4(double T0, double G) {
+    const double q1 = T0 * relax_factor(G, T0);
+    const double T1 = T0 + q1 / 2.0;
+    const double q2 = T1 * relax_factor(G, T1);
+    const double T2 = T0 + q2 / 2.0;
+    const double q3 = T2 * relax_factor(G, T2);
+    const double T3 = T0 + q3;
+    const double q4 = T3 * relax_factor(G, T3);
+    return T0 + (q1 + 2.0 * q2 + 2.0 * q3 + q4) / 6.0;
 }
 
 __device__ __forceinline__ double dte_G(int c, double h, const Mesh& M) {
@@ -1095,7 +1097,7 @@ __device__ __forceinline__ double dte_G(int c, double h, const Mesh& M) {
 }
 
 __device__ __forceinline__ void dte_B(int c, double h, const Mesh& M) {
-    M.Te[c] = dte_rk4(M.Te[c], dte_G(c, h, M), h);
+    M.Te[c] = dte_rk4(M.Te[c], dte_G(c, h, M));
 }
 
 __global__ __launch_bounds__(256) void k_dte_ab(double h, Mesh M, double* partials) {
@@ -1214,10 +1216,10 @@ __global__ __launch_bounds__(256) void k_dte_finish_ab(double h, Mesh M, const d
         const double G = dte_G(c, h, M);
         const double dE = dte_dE(M, c);
         double T = ee_to_te(M.Ee[c] * f);
-        T = dte_rk4(T, G, h);                  // B(h) ends the sub-step
+        T = dte_rk4(T, G);                  // B(h) ends the sub-step
         double ee = te_to_ee(T) + h * dE;      // A(h) ends it
         ee += h * dE;                          // A(h) starts the next
-        T = dte_rk4(ee_to_te(ee), G, h);       // B(h)
+        T = dte_rk4(ee_to_te(ee), G);       // B(h)
         ee = te_to_ee(T);                      // E(Te) as C starts
         M.Te[c] = T;
         M.Ee[c] = ee;
