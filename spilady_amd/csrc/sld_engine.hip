@@ -1249,6 +1249,7 @@ int sld_create_batch(int nrep, const sld_params* params, const sld_atoms* init, 
 int sld_nreplicas(const sld_ctx* c) { return c ? c->R : 0; }
 
 int sld_select_replica(sld_ctx* c, int r) {
+    if (!c) return SLD_ERR_ARG;
     if (r < 0 || r >= c->R) return set_err(c, SLD_ERR_ARG, "replica %d out of range (%d replicas)", r, c->R);
     c->sel = r;
     return SLD_OK;
@@ -1267,12 +1268,22 @@ int sld_destroy(sld_ctx* c) {
 
 const char* sld_last_error(const sld_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-int sld_links(sld_ctx* c) { TRY(do_links(c)); return read_flags(c); }
-int sld_calculate_rho(sld_ctx* c) { TRY(nbuild(c, 1)); return read_flags(c); }
-int sld_calculate_force_energy(sld_ctx* c) { TRY(do_force(c)); return read_flags(c); }
-int sld_core_ds(sld_ctx* c, double dt) { TRY(do_ds(c, dt)); TRY(sync_pass(c)); return read_flags(c); }
-int sld_core_dp(sld_ctx* c, double dt) { TRY(do_dp(c, dt)); TRY(sync_pass(c)); return read_flags(c); }
-int sld_core_dr(sld_ctx* c, double dt) { TRY(do_dr(c, dt)); return read_flags(c); }
+int sld_links(sld_ctx* c) { if (!c) return SLD_ERR_ARG; TRY(do_links(c)); return read_flags(c); }
+int sld_calculate_rho(sld_ctx* c) { if (!c) return SLD_ERR_ARG; TRY(nbuild(c, 1)); return read_flags(c); }
+int sld_calculate_force_energy(sld_ctx* c) { if (!c) return SLD_ERR_ARG; TRY(do_force(c)); return read_flags(c); }
+int sld_core_ds(sld_ctx* c, double dt) {
+    if (!c) return SLD_ERR_ARG;
+    TRY(do_ds(c, dt));
+    TRY(sync_pass(c));
+    return read_flags(c);
+}
+int sld_core_dp(sld_ctx* c, double dt) {
+    if (!c) return SLD_ERR_ARG;
+    TRY(do_dp(c, dt));
+    TRY(sync_pass(c));
+    return read_flags(c);
+}
+int sld_core_dr(sld_ctx* c, double dt) { if (!c) return SLD_ERR_ARG; TRY(do_dr(c, dt)); return read_flags(c); }
 
 int sld_core_dTe(sld_ctx* c, double dt) {
     if (!c->phys.eltemp) return set_err(c, SLD_ERR_STATE, "core_dTe needs eltemp");
@@ -1287,6 +1298,7 @@ int sld_calculate_temperature(sld_ctx* c) {
 }
 
 int sld_check(sld_ctx* c, int step, sld_obs* out) {
+    if (!c) return SLD_ERR_ARG;
     TRY(do_check(c, step, c->d_obs));
     if (out) HIPCHK(c, hipMemcpyAsync(out, c->d_obs, sizeof(sld_obs) * c->R, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(&c->last_obs, c->d_obs, sizeof(sld_obs), hipMemcpyDeviceToHost, c->stream));
@@ -1295,6 +1307,7 @@ int sld_check(sld_ctx* c, int step, sld_obs* out) {
 }
 
 int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
+    if (!c) return SLD_ERR_ARG;
     if (nsteps < 0) return set_err(c, SLD_ERR_ARG, "nsteps < 0");
     int every = c->prm.interval_of_print_out > 0 ? c->prm.interval_of_print_out : 1;
     int nprint = 0;
@@ -1345,23 +1358,27 @@ int sld_step(sld_ctx* c, int first_step, int nsteps, sld_obs* obs, int* nobs) {
 }
 
 int sld_last_step_timing(sld_ctx* c, double* total_ms) {
+    if (!c) return SLD_ERR_ARG;
     if (total_ms) *total_ms = c->last_ms;
     return SLD_OK;
 }
 
 int sld_time_kernel(sld_ctx* c, const char* kernel) {
+    if (!c) return SLD_ERR_ARG;
     c->timed_name = kernel ? kernel : "";
     c->ev_used = 0;
     return SLD_OK;
 }
 
 int sld_kernel_timing(sld_ctx* c, int* launches, double* total_ms) {
+    if (!c) return SLD_ERR_ARG;
     if (launches) *launches = c->timed_launches;
     if (total_ms) *total_ms = c->timed_ms;
     return SLD_OK;
 }
 
 int sld_download(sld_ctx* c, sld_atoms* o) {
+    if (!c) return SLD_ERR_ARG;
     if (!o) return SLD_ERR_ARG;
     Dev& D = c->D;
     const int r = c->sel;
@@ -1383,6 +1400,7 @@ int sld_download(sld_ctx* c, sld_atoms* o) {
 }
 
 int sld_upload(sld_ctx* c, const sld_atoms* in) {
+    if (!c) return SLD_ERR_ARG;
     if (!in) return SLD_ERR_ARG;
     Dev& D = c->D;
     const int r = c->sel;
@@ -1397,6 +1415,7 @@ int sld_upload(sld_ctx* c, const sld_atoms* in) {
 }
 
 int sld_get_cells(sld_ctx* c, sld_cells* o) {
+    if (!c) return SLD_ERR_ARG;
     if (!o) return SLD_ERR_ARG;
     Dev& D = c->D;
     const Rep& R = c->reps[c->sel];
@@ -1419,6 +1438,7 @@ int sld_set_cells_te(sld_ctx* c, const double* te) {
 }
 
 int sld_get_geometry(sld_ctx* c, int* ncell3, int* ngroups, int* group_sizes) {
+    if (!c) return SLD_ERR_ARG;
     const ReplicaHost& h = c->rh[c->sel];
     if (ncell3) { ncell3[0] = h.hc.nx; ncell3[1] = h.hc.ny; ncell3[2] = h.hc.nz; }
     if (ngroups) *ngroups = (int)h.group_size.size();
@@ -1427,6 +1447,7 @@ int sld_get_geometry(sld_ctx* c, int* ncell3, int* ngroups, int* group_sizes) {
 }
 
 int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
+    if (!c) return SLD_ERR_ARG;
     const Rep& R = c->reps[c->sel];
     const int C = R.ncell;
     std::vector<int> cn(C), cl((size_t)C * kCellCap);
@@ -1446,7 +1467,7 @@ int sld_get_cell_lists(sld_ctx* c, int cap, int* counts, int* lists) {
     return SLD_OK;
 }
 
-double sld_total_time(const sld_ctx* c) { return c->total_time; }
+double sld_total_time(const sld_ctx* c) { if (!c) return 0.0; return c->total_time; }
 
 int sld_scale_step(sld_ctx* c) {
     if (!c) return SLD_ERR_ARG;

@@ -153,3 +153,25 @@ def test_batch_rejects_mismatched_or_eltemp_replicas():
     rc = lib.sld_create_batch(2, pv, av, 0, C.byref(h))
     assert rc == 1 and b"changestep" in lib.sld_last_error(h)
     lib.sld_destroy(h)
+
+
+def test_entry_points_refuse_a_null_context():
+    """Every entry point that takes a context returns SLD_ERR_ARG (or 0 for
+    the time accessors) for a null one, without touching a device."""
+    lib = engine.load()
+    null = C.c_void_p(None)
+    buf = (C.c_double * 8)()
+    assert lib.sld_links(null) == 1
+    assert lib.sld_calculate_rho(null) == 1
+    assert lib.sld_calculate_force_energy(null) == 1
+    assert lib.sld_core_ds(null, C.c_double(1e-15)) == 1
+    assert lib.sld_core_dr(null, C.c_double(1e-15)) == 1
+    assert lib.sld_check(null, 0, None) == 1
+    assert lib.sld_step(null, 0, 1, None, None) == 1
+    assert lib.sld_get_box(null, buf) == 1
+    assert lib.sld_select_replica(null, 0) == 1
+    assert lib.sld_set_cells_te(null, buf) == 1
+    assert lib.sld_destroy(null) == 0
+    assert lib.sld_nreplicas(null) == 0
+    lib.sld_total_time.restype = C.c_double
+    assert lib.sld_total_time(null) == 0.0
