@@ -186,6 +186,15 @@ void ref_quantum_H(double* out) {
     (void)out;
 #endif
 }
+// write_vsim(step) (write_vsim.cpp:33-134; the shipped control.h has writevsim)
+void ref_write_vsim(int cur) {
+#ifdef writevsim
+    Quiet q(true);
+    write_vsim(cur);
+#else
+    (void)cur;
+#endif
+}
 void ref_get_box(double* b) { b[0] = d.xx; b[1] = d.yx; b[2] = d.yy; b[3] = d.zx; b[4] = d.zy; b[5] = d.zz; }
 
 int ref_natom(void) { return natom; }

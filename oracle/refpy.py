@@ -30,7 +30,8 @@ REF_KEYS = ["OMP_threads", "out_body", "step", "interval_of_print_out",
             "min_length_link_cell", "displace_limit", "phi_limit",
             "interval_of_scale_stress", "stress_xx", "stress_yx", "stress_yy",
             "stress_zx", "stress_zy", "stress_zz", "interval_of_scale_pressure",
-            "pressure", "baro_damping_time", "Msteps_quantum", "Nfrequency_quantum"]
+            "pressure", "baro_damping_time", "Msteps_quantum", "Nfrequency_quantum",
+            "interval_of_vsim", "vsim_prec"]
 
 # Fixed switches of the reference build (shipped control.h).
 REF_SWITCHES = {"spinlang": 1, "lattlang": 1, "eltemp": 1, "localcolmot": 1,
@@ -84,6 +85,7 @@ class Reference:
         L.ref_check.argtypes = [C.c_int]
         L.ref_scale.argtypes = [C.c_int]
         L.ref_get_box.argtypes = [_dp]
+        L.ref_write_vsim.argtypes = [C.c_int]
         L.ref_quantum_H.argtypes = [_dp]
         for nm in ("ref_step", "ref_total_time", "ref_virial", "ref_initial_ave_energy"):
             getattr(L, nm).restype = C.c_double

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <sstream>
 #include <string>
@@ -60,7 +61,9 @@ struct Vars {
         // quantum thermostat: control.h quantumnoise, default.h:222-225
         {"quantumnoise", "0"}, {"Msteps_quantum", "50"}, {"Nfrequency_quantum", "150"},
         // control.h readTe: the cells' electron temperatures from a cel- file (default.h:83-85)
-        {"readTe", "0"}, {"in_eltemp", "cel-input.dat"}};
+        {"readTe", "0"}, {"in_eltemp", "cel-input.dat"},
+        // control.h writevsim: V_sim files (default.h:67-69)
+        {"writevsim", "0"}, {"interval_of_vsim", "1000"}, {"vsim_prec", "4"}};
     void read(const char* path) {
         std::ifstream in(path);
         if (!in) fail(std::string("cannot open ") + path);
@@ -78,8 +81,10 @@ struct Atoms {
     double total_time = 0.0;
     double box[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> r, p, f, s, hh, hl, stress, rho, pe, ke, me, me0, vir, sjs;
+    std::vector<std::string> el;  // element column of the con- file
     void resize(int natom) {
         n = natom;
+        el.assign(n, "Fe");
         for (auto* v : {&r, &p, &f, &s, &hh, &hl}) v->assign((size_t)3 * n, 0.0);
         for (auto* v : {&rho, &pe, &ke, &me, &me0, &vir, &sjs}) v->assign(n, 0.0);
         stress.assign((size_t)6 * n, 0.0);
@@ -104,9 +109,8 @@ Atoms read_con(const std::string& path) {
     a.resize(n);
     for (int i = 0; i < n; ++i) {
         int idx;
-        std::string el;
         double s0, lv, hx, hy, hz, lx, ly, lz;
-        in >> idx >> el;
+        in >> idx >> a.el[i];
         for (int k = 0; k < 3; ++k) in >> a.r[3 * i + k];
         for (int k = 0; k < 3; ++k) in >> a.p[3 * i + k];
         for (int k = 0; k < 3; ++k) in >> a.f[3 * i + k];
@@ -189,10 +193,11 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
         const double* hl = &a.hl[3 * i];
         const double s0 = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
         std::fprintf(f,
-                     "%d Fe %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e "
+                     "%d %s %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e "
                      "%.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e %.15e "
                      "%.15e %.15e\n",
-                     i, r[0], r[1], r[2], p[0], p[1], p[2], fo[0], fo[1], fo[2], st[0], st[1], st[2], st[3], st[4],
+                     i, a.el[i].c_str(), r[0], r[1], r[2], p[0], p[1], p[2], fo[0], fo[1], fo[2], st[0], st[1], st[2],
+                     st[3], st[4],
                      st[5], a.rho[i], a.ke[i], a.pe[i], s[0], s[1], s[2], s0, hh[0], hh[0], hh[2], hl[0], hl[0],
                      hl[2], a.me[i], a.me0[i], lv);
     }
@@ -222,12 +227,42 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
 
 }  // namespace
 
+// write_vsim (write_vsim.cpp:33-134): V_sim files vsm-<body>_NNNN.ascii (the
+// box, then "x y z element px py pz"), .spin ("i |s| theta phi" in degrees)
+// and .dat (pe), scientific with vsim_prec digits
+void write_vsim(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a, int prec) {
+    sld_atoms v = a.view();
+    if (sld_download(ctx, &v)) fail(sld_last_error(ctx));
+    if (sld_get_box(ctx, a.box)) fail(sld_last_error(ctx));
+    const double t = sld_total_time(ctx);
+    const std::string stem = numbered("vsm-", body, n_output);
+    const std::string base = stem.substr(0, stem.size() - 4);  // without ".dat"
+    std::ofstream fa(base + ".ascii"), fs(base + ".spin"), fc(base + ".dat");
+    if (!fa || !fs || !fc) fail("cannot write the vsm- files");
+    for (std::ofstream* o : {&fa, &fs, &fc}) *o << std::scientific << std::setprecision(prec);
+    const double* d = a.box;
+    fa << a.n << " " << t << '\n' << d[0] << " " << d[1] << " " << d[2] << '\n' << d[3] << " " << d[4] << " " << d[5]
+       << '\n';
+    fs << a.n << '\n';
+    const double pi = 3.141592653589793238462e0;
+    for (int i = 0; i < a.n; ++i) {
+        const double* r = &a.r[3 * i];
+        const double* p = &a.p[3 * i];
+        const double* s = &a.s[3 * i];
+        fa << r[0] << " " << r[1] << " " << r[2] << " " << a.el[i] << " " << p[0] << " " << p[1] << " " << p[2] << '\n';
+        const double ss = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        fs << i + 1 << " " << ss << " " << std::acos(s[2] / ss) * 180e0 / pi << " " << std::atan2(s[1], s[0]) * 180e0 / pi
+           << '\n';
+        fc << a.pe[i] << '\n';
+    }
+}
+
 // variables.all (read_variables.cpp:278-418): the variables in effect, in
 // the reference's order and with the stream's default formatting, for the
 // switches this driver runs. OMP_threads is echoed as the reference's OpenMP
-// build does; the V_sim keys are left out, since no V_sim files are written.
+// build does.
 void write_variables_all(const Vars& v, const sld_params& prm, const sld_barostat& baro, bool runstep,
-                         bool read_te) {
+                         bool read_te, bool vsim) {
     std::ofstream out("variables.all");
     if (!out) fail("cannot write variables.all");
     out << "OMP_threads " << v.i("OMP_threads") << '\n';
@@ -237,6 +272,10 @@ void write_variables_all(const Vars& v, const sld_params& prm, const sld_barosta
     out << "step " << prm.step << '\n';
     out << "interval_of_print_out " << prm.interval_of_print_out << '\n';
     out << "interval_of_config_out " << v.i("interval_of_config_out") << '\n';
+    if (vsim) {
+        out << "interval_of_vsim " << v.i("interval_of_vsim") << '\n';
+        out << "vsim_prec " << v.i("vsim_prec") << '\n';
+    }
     out << "in_config " << v.s("in_config") << '\n';
     if (read_te) out << "in_eltemp " << v.s("in_eltemp") << '\n';
     if (prm.eltemp) out << "kappa_e " << prm.kappa_e << '\n';
@@ -316,7 +355,10 @@ int main(int argc, char** argv) {
     baro.baro_damping_time = v.d("baro_damping_time");
     const bool barostats = baro.stress_on || baro.pressure_on;
     const bool runstep = v.i("runstep") != 0;
-    write_variables_all(v, prm, baro, runstep, prm.eltemp && v.i("readTe"));
+    const bool vsim = v.i("writevsim") != 0;
+    const int every_vsim = (int)std::max(1LL, v.i("interval_of_vsim"));
+    const int vsim_prec = (int)v.i("vsim_prec");
+    write_variables_all(v, prm, baro, runstep, prm.eltemp && v.i("readTe"), vsim);
     const double total_production_time = v.d("total_production_time");
     const std::string body = v.s("out_body");
     const long long nsteps = v.i("no_of_production_steps");
@@ -394,6 +436,7 @@ int main(int argc, char** argv) {
     if (sld_get_box(ctx, box)) fail(sld_last_error(ctx));
     append_logs(body, o, prm, box);
     write_config(ctx, body, 0, a, prm.eltemp);
+    if (vsim) write_vsim(ctx, body, 0, a, vsim_prec);
 
     std::vector<sld_obs> obs;
     if (!runstep) {
@@ -410,8 +453,10 @@ int main(int argc, char** argv) {
             std::printf("%lld total time = %.6e dt = %.3e\n", s, sld_total_time(ctx), sld_get_step(ctx));
             ++s;
             if (s % every_config == 0) write_config(ctx, body, (int)((s - 1) / every_config + 1), a, prm.eltemp);
+            if (vsim && s % every_vsim == 0) write_vsim(ctx, body, (int)((s - 1) / every_vsim + 1), a, vsim_prec);
         }
         write_config(ctx, body, 9999, a, prm.eltemp);  // write(-2)
+        if (vsim) write_vsim(ctx, body, 9999, a, vsim_prec);
         sld_destroy(ctx);
         std::printf("SPILADY (libspilady_hip) ends\n");
         return 0;
@@ -422,6 +467,7 @@ int main(int argc, char** argv) {
     // and its box is logged with its check
     for (long long s = 0; s < nsteps;) {
         long long next = (s / every_config + 1) * every_config;
+        if (vsim) next = std::min(next, (s / every_vsim + 1) * every_vsim);
         if (next > nsteps) next = nsteps;
         if (barostats) next = s + 1;
         const int n = (int)(next - s);
@@ -435,8 +481,10 @@ int main(int argc, char** argv) {
         }
         s = next;
         if (s % every_config == 0) write_config(ctx, body, (int)((s - 1) / every_config + 1), a, prm.eltemp);
+        if (vsim && s % every_vsim == 0) write_vsim(ctx, body, (int)((s - 1) / every_vsim + 1), a, vsim_prec);
     }
     write_config(ctx, body, 9999, a, prm.eltemp);  // write(-2)
+    if (vsim) write_vsim(ctx, body, 9999, a, vsim_prec);
     sld_destroy(ctx);
     std::printf("SPILADY (libspilady_hip) ends\n");
     return 0;

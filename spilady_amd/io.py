@@ -74,6 +74,10 @@ VARIABLE_DEFAULTS: Dict[str, object] = {
     # control.h readTe: cell electron temperatures from a cel- file (default.h:83-85)
     "readTe": 0,
     "in_eltemp": "cel-input.dat",
+    # control.h writevsim: V_sim files vsm-<body>_NNNN.ascii/.spin/.dat (default.h:67-69)
+    "writevsim": 0,
+    "interval_of_vsim": 1000,
+    "vsim_prec": 4,
 }
 
 

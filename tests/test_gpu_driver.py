@@ -239,9 +239,8 @@ def test_driver_reads_cell_temperatures(tmp_path):
 
 def test_driver_variables_all_matches_reference(tmp_path):
     """variables.all (read_variables.cpp:278-418): for tests/golden/variables_in.txt
-    (the shipped control.h's switches) the driver's echo equals the one the
-    compiled reference wrote, line for line, except the two V_sim keys (this
-    build writes no V_sim files)."""
+    (the shipped control.h's switches, writevsim included) the driver's echo
+    equals the one the compiled reference wrote, line for line."""
     from spilady_amd import abi
     tmp = str(tmp_path)
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -259,8 +258,39 @@ def test_driver_variables_all_matches_reference(tmp_path):
     with open(os.path.join(tmp, "variables.all")) as fh:
         ours = fh.read().splitlines()
     with open(os.path.join(gold, "variables_all_ref.txt")) as fh:
-        ref = [ln for ln in fh.read().splitlines() if ln.split()[0] not in ("interval_of_vsim", "vsim_prec")]
+        ref = fh.read().splitlines()
     assert ours == ref
+
+
+def test_driver_vsim_matches_reference(tmp_path):
+    """writevsim (write_vsim.cpp:33-134): from the configuration of
+    tests/golden/vsim_fe1k.npz and the variables of variables_in.txt, the
+    driver's vsm-drv_0000.ascii/.spin/.dat equal the files the compiled
+    reference wrote, character for character; later files follow the
+    interval (steps 2, 4 of 5) and the final state (_9999)."""
+    tmp = str(tmp_path)
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = np.load(os.path.join(gold, "vsim_fe1k.npz"), allow_pickle=False)
+    with open(os.path.join(gold, "variables_in.txt")) as fh:
+        text = fh.read()
+    v = io.read_variables(os.path.join(gold, "variables_in.txt"))
+    with open(os.path.join(tmp, "variables.in"), "w") as out:
+        out.write(text + "runstep 1\nno_of_production_steps 5\n")
+    state = io.AtomState(box=g["box"], **{k: g[k] for k in ("r", "p", "f", "s", "stress", "rho", "ke", "pe",
+                                                             "me", "me0")})
+    io.write_con(os.path.join(tmp, v["in_config"]), state)
+    res = subprocess.run([DRIVER, "variables.in"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    for ext in ("ascii", "spin", "dat"):
+        with open(os.path.join(tmp, f"vsm-drv_0000.{ext}")) as fh:
+            assert fh.read() == str(g[f"vsm_{ext}"]), ext
+    names = sorted(f for f in os.listdir(tmp) if f.startswith("vsm-drv_") and f.endswith(".ascii"))
+    assert names == ["vsm-drv_0000.ascii", "vsm-drv_0001.ascii", "vsm-drv_0002.ascii", "vsm-drv_9999.ascii"]
+    fin = io.read_con(os.path.join(tmp, "con-drv_9999.dat"))
+    with open(os.path.join(tmp, "vsm-drv_9999.ascii")) as fh:
+        rows = fh.read().splitlines()[3:]
+    assert len(rows) == fin.natom
+    assert [float(x) for x in rows[5].split()[:3]] == [float("%.4e" % x) for x in fin.r[5]]
 
 
 def test_driver_checkpoint_resume(tmp_path):
