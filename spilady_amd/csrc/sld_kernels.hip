@@ -667,7 +667,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     const double rs = lang ? sqrt(2.0 * Tk * gS / dt) : 0.0;
     auto atom_at = [&](int q) { return lst[backward ? n - 1 - q : q]; };
 
-    // pipeline registers: atom q (_c) and atom q+1 (_n)
+    // pipeline registers: atom q (_c), atom q+1 (_n) and atom q+2's list entry
     int a_c = atom_at(0), nm_c = D.nbmn[a_c];
     int id_c[kPipe], id_n[kPipe];
     double J_c[kPipe];
@@ -689,6 +689,9 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
         load_ids(D, a_n, id_n);
         rho_n = lang ? D.rho[a_n] : 0.0;
     }
+    // the list entry two atoms ahead, so that no iteration waits for one
+    // before it can request the next atom's ids
+    int a_nn = n > 2 ? atom_at(2) : a_c;
     int prev_a = -1;
     double4 prev_s = make_double4(0.0, 0.0, 0.0, 0.0);
     double4 prev_old = prev_s;  // the previous atom's spin before its update
@@ -738,10 +741,11 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             a_c = a_n;
             nm_c = nm_n;
             if (q + 2 < n) {
-                a_n = atom_at(q + 2);
+                a_n = a_nn;
                 nm_n = D.nbmn[a_n];
                 load_ids(D, a_n, id_n);
                 if (lang) rho_n = D.rho[a_n];
+                if (q + 3 < n) a_nn = atom_at(q + 3);
             }
         }
         double3 L = make_double3(0.0, 0.0, 0.0);
