@@ -467,8 +467,8 @@ int do_dTe_launches(sld_ctx* c, double dt) {
                 double* part = dir == 1 ? c->d_part + (size_t)nb + npb : nullptr;
                 const int bs = c->shapes.sweep_block;
                 const Box& bx = c->reps[0].box;
-                LAUNCH(c, k_dte_sweep, nblk(n, bs), bs, L.d_cells + L.off[l], n, pref, h0.dx2,
-                       h0.dy2, h0.dz2, bx.nx, bx.ny, bx.nz, c->D.Te, c->D.Ee, part);
+                LAUNCH(c, k_dte_sweep, nblk(n, bs), bs, L.d_cells + L.off[l], n, pref, 1.0 / h0.dx2,
+                       1.0 / h0.dy2, 1.0 / h0.dz2, bx.nx, bx.ny, bx.nz, c->D.Te, c->D.Ee, part);
                 if (dir == 1) npb += nblk(n, 64);  // one partial per wave
                 nlaunch += n > 0;
             }

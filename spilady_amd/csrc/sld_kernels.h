@@ -102,8 +102,10 @@ __global__ void k_temperature(Tables T, Phys P, Dev D);
 __global__ void k_dte_ab(double h, Mesh M, double* partials);
 // the arguments are the few the stencil reads: one of these launches per
 // level and sub-step, 3,600 per 2M-atom step
-__global__ void k_dte_sweep(const int* cells, int ncell, double pref, double dx2, double dy2,
-                            double dz2, int nx, int ny, int nz, double* Te, double* Ee, double* partials);
+// idx2, idy2, idz2: reciprocals of the squared cell edges (the stencil
+// multiplies by them where the reference divides)
+__global__ void k_dte_sweep(const int* cells, int ncell, double pref, double idx2, double idy2,
+                            double idz2, int nx, int ny, int nz, double* Te, double* Ee, double* partials);
 __global__ void k_reduce2(const double* pa, int na, const double* pb, int nb, double* out);
 __global__ void k_dte_finish(double h, Mesh M, const double* xp, int nxp);
 __global__ void k_dte_finish_ab(double h, Mesh M, const double* xp, int nxp, double* partials);

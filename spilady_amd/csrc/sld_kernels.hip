@@ -1124,8 +1124,8 @@ __global__ __launch_bounds__(256) void k_dte_ab(double h, Mesh M, double* partia
 // In the backward sweep every cell is updated exactly once, last, so the
 // launch also leaves partials of the final energies (partials != null), one
 // per wave of 64 level entries: the sums do not depend on the block size.
-__global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref, double dx2,
-                                                   double dy2, double dz2, int nx, int ny, int nz, double* Te,
+__global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref, double idx2,
+                                                   double idy2, double idz2, int nx, int ny, int nz, double* Te,
                                                    double* Ee, double* partials) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
@@ -1142,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, 
         const double tym = Te[ix + ym * nx + iz * sl], typ = Te[ix + yp * nx + iz * sl];
         const double tzm = Te[ix + iy * nx + zm * sl], tzp = Te[ix + iy * nx + zp * sl];
         double ee = Ee[c];
-        ee += pref * ((txp + txm - 2.0 * tc) / dx2 + (typ + tym - 2.0 * tc) / dy2 + (tzp + tzm - 2.0 * tc) / dz2);
+        ee += pref * ((txp + txm - 2.0 * tc) * idx2 + (typ + tym - 2.0 * tc) * idy2 + (tzp + tzm - 2.0 * tc) * idz2);
         Ee[c] = ee;
         Te[c] = ee_to_te(ee);
         acc[0] = ee;
