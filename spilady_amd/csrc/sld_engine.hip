@@ -839,6 +839,29 @@ int upload_levels(sld_ctx* c, LevelSet& L) {
     return SLD_OK;
 }
 
+// The electron sweep reads its cells as packed coordinates ix | iy << 10 |
+// iz << 20, which spares the stencil three integer divisions. The electron
+// mesh exists on single-system contexts only; batched ones upload plain
+// indices, which no sweep reads.
+int upload_dte_levels(sld_ctx* c, LevelSet& L) {
+    if (c->R == 1) {
+        const HostCells& hc = c->rh[0].hc;
+        if (hc.nx > 1023 || hc.ny > 1023 || hc.nz > 1023)
+            return set_err(c, SLD_ERR_CAPACITY, "the electron mesh needs at most 1023 link cells per side");
+        std::vector<int> packed(L.cells.size());
+        for (size_t k = 0; k < L.cells.size(); ++k) {
+            const int cell = L.cells[k];
+            const int ix = cell % hc.nx, iy = (cell / hc.nx) % hc.ny, iz = cell / (hc.nx * hc.ny);
+            packed[k] = ix | (iy << 10) | (iz << 20);
+        }
+        TRY(upload_vec(c, &L.d_cells, packed));
+    } else {
+        TRY(upload_vec(c, &L.d_cells, L.cells));
+    }
+    TRY(upload_vec(c, &L.d_off, L.off));
+    return SLD_OK;
+}
+
 // the level schedules of all replicas as one: level l holds every replica's
 // level-l cells (replicas never interact)
 void merge_levels(const std::vector<LevelSet>& per, const std::vector<Rep>& reps, LevelSet& out) {
@@ -1158,8 +1181,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     merge_levels(tb, c->reps, c->dte_bwd);
     TRY(upload_levels(c, c->spin_fwd));
     TRY(upload_levels(c, c->spin_bwd));
-    TRY(upload_levels(c, c->dte_fwd));
-    TRY(upload_levels(c, c->dte_bwd));
+    TRY(upload_dte_levels(c, c->dte_fwd));
+    TRY(upload_dte_levels(c, c->dte_bwd));
 
     // electron temperatures (map_cells.cpp:128-159)
     if (P0.eltemp) {

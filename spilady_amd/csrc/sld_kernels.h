@@ -102,6 +102,7 @@ __global__ void k_temperature(Tables T, Phys P, Dev D);
 __global__ void k_dte_ab(double h, Mesh M, double* partials);
 // the arguments are the few the stencil reads: one of these launches per
 // level and sub-step, 3,600 per 2M-atom step
+// cells: the level's cells as packed coordinates ix | iy << 10 | iz << 20;
 // idx2, idy2, idz2: reciprocals of the squared cell edges (the stencil
 // multiplies by them where the reference divides)
 __global__ void k_dte_sweep(const int* cells, int ncell, double pref, double idx2, double idy2,

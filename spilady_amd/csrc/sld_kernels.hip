@@ -1130,9 +1130,10 @@ __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (t < ncell) {
-        const int c = cells[t];
+        const int pk = cells[t];  // ix | iy << 10 | iz << 20
         const int sl = nx * ny;
-        const int ix = c % nx, iy = (c / nx) % ny, iz = c / sl;
+        const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = pk >> 20;
+        const int c = ix + iy * nx + iz * sl;
         const int xm = ix == 0 ? nx - 1 : ix - 1, xp = ix + 1 == nx ? 0 : ix + 1;
         const int ym = iy == 0 ? ny - 1 : iy - 1, yp = iy + 1 == ny ? 0 : iy + 1;
         const int zm = iz == 0 ? nz - 1 : iz - 1, zp = iz + 1 == nz ? 0 : iz + 1;
