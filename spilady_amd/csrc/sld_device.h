@@ -179,9 +179,13 @@ __device__ __forceinline__ double te_to_ee(double te) {
     const double e = exp(b * te);
     return 3.0 / b * log(0.5 * (e + 1.0 / e));
 }
+// T(E) = acosh(exp(bE/3)) / b (heatcapacity_CPU.cpp:34-57), written as
+// acosh(e^y) = y + log1p(sqrt(1 - e^-2y)) with 1 - e^-2y = -expm1(-2y):
+// fewer instructions than exp and acosh, and no loss near y = 0
 __device__ __forceinline__ double ee_to_te(double ee) {
     const double b = 2e-4 / kBoltz;
-    return acosh(exp(b / 3.0 * ee)) / b;
+    const double y = b / 3.0 * ee;
+    return (y + log1p(sqrt(-expm1(-2.0 * y)))) / b;
 }
 
 }  // namespace sld
