@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "heatcap_poly.h"
 #include "sld_common.h"
 
 namespace sld {
@@ -170,22 +171,39 @@ __device__ __forceinline__ double3 normal3(uint64_t seed, int replica, int atom,
 }
 
 // ---------------------------------------------------------------------------
-// electron heat capacity C = 3 tanh(bT), b = 2e-4/k_B (heatcapacity_CPU.cpp:34-57)
-__device__ __forceinline__ double ce_of(double te) { return 3.0 * tanh((2e-4 / kBoltz) * te); }
-// cosh through one exp and its reciprocal (about 60 instructions against
-// 142 for the library cosh), then the log
-__device__ __forceinline__ double te_to_ee(double te) {
-    const double b = 2e-4 / kBoltz;
-    const double e = exp(b * te);
-    return 3.0 / b * log(0.5 * (e + 1.0 / e));
+// electron heat capacity C = 3 tanh(bT), energy E = 3/b log cosh(bT) and its
+// inverse T = acosh(exp(bE/3))/b, b = 2e-4/k_B (heatcapacity_CPU.cpp:34-57).
+// For bT <= kPolyXMax (T <= 6250 K) they are evaluated as x^2 Q(x^2),
+// x R(x^2) and sqrt(2y) G(y) with the fitted polynomials of heatcap_poly.h
+// (about 1 ulp, a few dozen FMAs instead of exp, log, expm1 and log1p);
+// outside that range, and for NaN, through the library functions.
+template <int N>
+__device__ __forceinline__ double horner(const double (&c)[N], double t) {
+    double a = c[N - 1];
+#pragma unroll
+    for (int k = N - 2; k >= 0; --k) a = fma(a, t, c[k]);
+    return a;
 }
-// T(E) = acosh(exp(bE/3)) / b (heatcapacity_CPU.cpp:34-57), written as
-// acosh(e^y) = y + log1p(sqrt(1 - e^-2y)) with 1 - e^-2y = -expm1(-2y):
-// fewer instructions than exp and acosh, and no loss near y = 0
+constexpr double kCeB = 2e-4 / kBoltz;
+
+// tanh(x), x = bT, for the heat capacity 3 tanh(x)
+__device__ __forceinline__ double tanh_bt(double x) {
+    if (fabs(x) <= kPolyXMax) return x * horner(kTanhR, x * x);
+    const double v = expm1(-2.0 * x);
+    return -v / (2.0 + v);
+}
+// E(T): cosh through one exp and its reciprocal outside the fitted range
+__device__ __forceinline__ double te_to_ee(double te) {
+    const double x = kCeB * te;
+    if (fabs(x) <= kPolyXMax) return (3.0 / kCeB) * (x * x) * horner(kLogCoshQ, x * x);
+    const double e = exp(x);
+    return 3.0 / kCeB * log(0.5 * (e + 1.0 / e));
+}
+// T(E); outside the fitted range acosh(e^y) = y + log1p(sqrt(-expm1(-2y)))
 __device__ __forceinline__ double ee_to_te(double ee) {
-    const double b = 2e-4 / kBoltz;
-    const double y = b / 3.0 * ee;
-    return (y + log1p(sqrt(-expm1(-2.0 * y)))) / b;
+    const double y = kCeB / 3.0 * ee;
+    if (y <= kPolyYMax) return sqrt(2.0 * y) * horner(kAcoshExpG, y) * (1.0 / kCeB);
+    return (y + log1p(sqrt(-expm1(-2.0 * y)))) / kCeB;
 }
 
 }  // namespace sld

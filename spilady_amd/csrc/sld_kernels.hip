@@ -1069,13 +1069,9 @@ __device__ __forceinline__ void dte_A(int c, double h, const Mesh& M) {
     M.Te[c] = ee_to_te(ee);
 }
 
-// exp(-G / C(T)) - 1 with C(T) = 3 tanh(bT) (heatcapacity_CPU.cpp:34-57).
-// For T > 0, tanh(bT) = -v / (2 + v) with v = expm1(-2bT), so
-// -G / C = G (2 + v) / (3 v): one expm1 and one division, a third of the
-// instructions of tanh and a second division, to a few ulp.
+// exp(-G / C(T)) - 1 with C(T) = 3 tanh(bT) (heatcapacity_CPU.cpp:34-57)
 __device__ __forceinline__ double relax_factor(double G, double T) {
-    const double v = expm1((-2.0 * (2e-4 / kBoltz)) * T);
-    return exp(G * (2.0 + v) / (3.0 * v)) - 1.0;
+    return exp(-G / (3.0 * tanh_bt(kCeB * T))) - 1.0;
 }
 
 // RK4 of the electron-ion/spin relaxation over h (core_dTe_CPU.cpp:91-129),
