@@ -169,6 +169,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+static_assert(kCellCap <= 64, "k_nbuild holds a cell's own atoms one per lane");
+
+// lane l's value of v, as a wave-uniform double
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int write_rho) {
     __shared__ int s_cand[kNbWaves][kMaxCand];
     const int lane = threadIdx.x & 63;
@@ -217,9 +227,14 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
     const int nc = D.cn[c];
     const int* own = D.cl + c * kCellCap;
     bool over = false;
+    // the cell's own atoms (nc <= kCellCap <= 64): id and position one per
+    // lane, loaded once, then read at the loop's uniform index, so no atom
+    // waits for its own id and position before its distances
+    const int oi = lane < nc ? own[lane] : 0;
+    const double4 orv = lane < nc ? D.r[oi] : make_double4(0.0, 0.0, 0.0, 0.0);
     for (int k = 0; k < nc; ++k) {
-        const int i = own[k];
-        const double3 ri = d3(D.r[i]);
+        const int i = __builtin_amdgcn_readlane(oi, k);
+        const double3 ri = make_double3(readlane_f64(orv.x, k), readlane_f64(orv.y, k), readlane_f64(orv.z, k));
         int nn = 0, nm = 0;
         double rho = 0.0;
         int* nb = D.nb + (size_t)i * kMaxNb;
