@@ -509,8 +509,9 @@ __device__ __forceinline__ double3 landau_field(double3 s, double A, double B, d
     double s2 = sq3(s);
     double s4 = s2 * s2;
     double k = -(2.0 * A + 4.0 * B * s2 + 6.0 * C * s4 + 8.0 * Dd * s4 * s2);
-    double s0 = sqrt(s2);
-    double hc = s0 > 0.0 ? -sjs / s0 : 0.0;
+    // -sjs / |s| through one reciprocal square root (the reference's sqrt and
+    // division, to about an ulp, on a shorter dependent chain)
+    double hc = s2 > 0.0 ? -sjs * rsqrt(s2) : 0.0;
     return make_double3((k + hc) * s.x, (k + hc) * s.y, (k + hc) * s.z);
 }
 
