@@ -127,3 +127,24 @@ def workload(name: str) -> Workload:
                          nsteps=1000)
         return Workload(v, at, 69)
     raise KeyError(name)
+
+
+BATCH_SIZES = (8, 12, 16, 20, 24, 28, 32)
+
+
+def batch_replicas(nrep: int = 64, draw_seed: int = 3, *, nsteps: int = 2000):
+    """The batched-replica workload: nrep independent bcc Fe replicas of L^3
+    cubic cells with L drawn uniformly from {8,12,...,32} (a Philox-free
+    numpy draw keyed by draw_seed), temperatures evenly spaced over
+    100-1,400 K, replica r seeded 100 + r and given noise stream r, spins as
+    in the small plain run, Langevin lattice and spin thermostats, dt = 1 fs.
+    Returns [(variables, atoms)] in replica order."""
+    rng = np.random.default_rng(draw_seed)
+    sizes = rng.choice(list(BATCH_SIZES), size=nrep)
+    temps = np.linspace(100.0, 1400.0, nrep)
+    out = []
+    for r in range(nrep):
+        at = make_bcc_fe(int(sizes[r]), float(temps[r]), 100 + r)
+        v = fe_variables(float(temps[r]), 100 + r, thermostats=True, print_every=1, nsteps=nsteps)
+        out.append((v, at))
+    return out

@@ -17,6 +17,12 @@ __device__ __forceinline__ double3 sub3(double3 a, double3 b) { return make_doub
 __device__ __forceinline__ double3 mul3(double k, double3 a) { return make_double3(k * a.x, k * a.y, k * a.z); }
 __device__ __forceinline__ double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ double sq3(double3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+// |s|, the value every spin's .w payload holds. It is the only definition of
+// that value: the field sums read a neighbour's .w where the reference takes
+// vec_length(s_j) (calculate_spin_CPU.cpp:265), so every writer of .w must
+// round it the same way. The explicit fma order leaves the compiler no
+// contraction choice that could differ between kernels.
+__device__ __forceinline__ double spin_len(double3 s) { return sqrt(fma(s.z, s.z, fma(s.x, s.x, s.y * s.y))); }
 __device__ __forceinline__ double3 cross3(double3 a, double3 b) {
     return make_double3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
 }

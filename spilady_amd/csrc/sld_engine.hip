@@ -784,7 +784,9 @@ int collect_timing(sld_ctx* c) {
     return SLD_OK;
 }
 
-// host <-> device copies of replica `r`'s atom slots [atom0, atom0 + natom)
+// host <-> device copies of replica `r`'s atom slots [atom0, atom0 + natom).
+// With with_len (spins) the device sets .w = |s| afterwards, with the one
+// rounding every other writer of .w uses.
 int up4(sld_ctx* c, double4* d, const double* src, int r, bool with_len) {
     if (!src) return SLD_OK;
     const Rep& R = c->reps[r];
@@ -792,9 +794,10 @@ int up4(sld_ctx* c, double4* d, const double* src, int r, bool with_len) {
     for (int i = 0; i < R.natom; ++i) {
         double x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
         h[4 * i] = x; h[4 * i + 1] = y; h[4 * i + 2] = z;
-        h[4 * i + 3] = with_len ? std::sqrt(x * x + y * y + z * z) : 0.0;
+        h[4 * i + 3] = 0.0;
     }
     HIPCHK(c, hipMemcpyAsync(d + R.atom0, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (with_len) LAUNCH(c, k_spin_len, nblk(R.natom, 256), 256, c->D, R.atom0, R.natom);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SLD_OK;
 }

@@ -117,6 +117,7 @@ __global__ void k_reduce(const double* partials, int nblk, int nk, double* out);
 __global__ void k_reduce_seg(const double* partials, const int* seg, int nk, double* out, int out_off);
 __global__ void k_check_nonfinite(Dev D);
 __global__ void k_cells_ee(Dev D, int c0, int n);
+__global__ void k_spin_len(Dev D, int a0, int n);
 
 // ---- parametric potential model (sld_model.hip) ----
 struct ModelTab {

@@ -787,7 +787,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             s = apply_rot(R, s);
             D.sjs[a] = S;
         }
-        prev_s = make_double4(s.x, s.y, s.z, sqrt(sq3(s)));
+        prev_s = make_double4(s.x, s.y, s.z, spin_len(s));
         prev_a = a;
         D.s[a] = prev_s;
         D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
@@ -1038,10 +1038,13 @@ __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
         double3 L = make_double3((k + hc) * s.x, (k + hc) * s.y, (k + hc) * s.z);
         lup = sq3(add3(H, L));
         ldn = 6.0 * A + 20.0 * B * s2 + 42.0 * C * s4 + 72.0 * Dd * s4 * s2 + 2.0 * S / s0;
+        // s0 stays local: the reference stores it (calculate_temperature_CPU.cpp:100),
+        // but here .w is read by the other cells' lanes of this launch through
+        // exchange_field, so writing it would race with those reads. The spin
+        // sweep and the uploads already keep .w = spin_len(s).
         D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
         D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
         D.sjs[a] = S;
-        D.s[a].w = s0;
     }
     double sum_ke = seg_sum32(ke);
     double3 ap = seg_sum32(pa);
@@ -1418,6 +1421,13 @@ __global__ __launch_bounds__(256) void k_cells_ee(Dev D, int c0, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     D.Ee[c0 + k] = te_to_ee(D.Te[c0 + k]);
+}
+
+// .w = |s| over the atom slots [a0, a0 + n), after the host uploaded spins
+__global__ __launch_bounds__(256) void k_spin_len(Dev D, int a0, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    D.s[a0 + k].w = spin_len(d3(D.s[a0 + k]));
 }
 
 __global__ __launch_bounds__(256) void k_check_nonfinite(Dev D) {

@@ -46,3 +46,36 @@ def assert_close(a, b, name: str, tol: float = TOL):
     e = relerr(a, b)
     assert e <= tol, f"{name}: normwise relative error {e:.3e} > {tol:.1e}"
     return e
+
+
+def ulp_distance(a, b) -> np.ndarray:
+    """Distance in units in the last place between float64 arrays: the gap
+    between their bit patterns on the monotone integer line of doubles."""
+    ia = np.asarray(a, dtype=np.float64).view(np.int64)
+    ib = np.asarray(b, dtype=np.float64).view(np.int64)
+    # map sign-magnitude bit patterns onto one monotone integer scale
+    # (exact integers: the patterns of nearby doubles differ in bits a
+    # float64 difference would round away)
+    ia = [(-(2 ** 63) - int(v)) if v < 0 else int(v) for v in ia.ravel()]
+    ib = [(-(2 ** 63) - int(v)) if v < 0 else int(v) for v in ib.ravel()]
+    return np.array([float(abs(x - y)) for x, y in zip(ia, ib)])
+
+
+def bitwise_diffs(a, b, keys=None):
+    """Every array of two dicts that is not bitwise equal, as one line each
+    (elements that differ, largest ulp distance); empty when all are equal."""
+    out = []
+    for k in keys if keys is not None else sorted(a):
+        x, y = np.asarray(a[k]), np.asarray(b[k])
+        if x.shape != y.shape:
+            out.append(f"{k}: shapes {x.shape} and {y.shape}")
+            continue
+        same = (x == y) | (np.isnan(x) & np.isnan(y)) if x.dtype.kind == "f" else (x == y)
+        nd = int(np.size(same) - np.count_nonzero(same))
+        if nd and x.dtype.kind == "f":
+            u = float(np.max(ulp_distance(x[~same], y[~same])))
+            out.append(f"{k}: {nd} of {x.size} differ, largest distance {u:.0f} ulp")
+        elif nd:
+            u = int(np.max(np.abs(x[~same].astype(np.int64) - y[~same].astype(np.int64))))
+            out.append(f"{k}: {nd} of {x.size} differ, largest difference {u}")
+    return out

@@ -11,7 +11,9 @@ from oracle import orcpy
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0", "stress", "heff_h", "heff_l")
+FIELDS = ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0", "stress", "heff_h", "heff_l", "sum_jij_sj")
+# compared with the oracle: the state and the effective fields
+ORACLE_FIELDS = ("r", "p", "f", "s", "heff_h", "heff_l", "sum_jij_sj", "rho", "pe", "ke", "me", "me0")
 # bcc Fe unit cells per side, temperature (K), seed: 5, 6 and 4 link cells per
 # side, i.e. 20, 8 and 8 cell groups
 REPLICAS = [(8, 300.0, 101), (10, 900.0, 102), (7, 1400.0, 103)]
@@ -67,7 +69,7 @@ def test_batch_replica_matches_oracle():
     o.step(0, 8)
     b.select(1)
     x, y = o.download(), b.download()
-    for f in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+    for f in ORACLE_FIELDS:
         assert_close(x[f], y[f], f"replica 1 {f}", TOL)
     o.close()
     b.close()
@@ -91,7 +93,7 @@ def test_batch_quantum_noise_matches_oracle():
         o.step(0, 8)
         b.select(r)
         x, y = o.download(), b.download()
-        for f in ("r", "p", "f", "s", "rho", "pe", "me", "me0"):
+        for f in ORACLE_FIELDS:
             assert_close(x[f], y[f], f"replica {r} {f}", TOL)
         o.close()
     b.close()

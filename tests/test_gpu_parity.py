@@ -15,7 +15,11 @@ from spilady_amd import constants as K
 
 pytestmark = pytest.mark.gpu
 
-ATOM_FIELDS = ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0", "stress")
+# the effective fields are compared with the state: Heff_H and Heff_L as the
+# last spin update or temperature pass left them, and sum_j J|s_j|
+# (calculate_spin_CPU.cpp:111-194, calculate_temperature_CPU.cpp:94-110)
+ATOM_FIELDS = ("r", "p", "f", "s", "heff_h", "heff_l", "sum_jij_sj", "rho", "pe", "ke", "me", "me0",
+               "stress")
 CELL_FIELDS = ("te", "tl", "ts_r", "ts_l", "gel", "ges")
 
 
