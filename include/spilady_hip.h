@@ -203,6 +203,18 @@ int sld_set_barostat(sld_ctx* ctx, const sld_barostat* b);
  * then changestep. It uses the stresses and pressure of the last check().
  * sld_step calls it after every step itself. */
 int sld_scale(sld_ctx* ctx, int step);
+/* The two barostats one at a time, for callers that run scale() themselves:
+ * scale_stress (scale_stress_CPU.cpp:33-86) and scale_pressure
+ * (scale_pressure_CPU.cpp:33-61). Each acts unconditionally (the caller
+ * keeps the interval test of scale.cpp:33-39) on the stresses or pressure of
+ * the last sld_check, over the time since that barostat last acted. */
+int sld_scale_stress(sld_ctx* ctx);
+int sld_scale_pressure(sld_ctx* ctx);
+/* Sets the context's clock, the reference's global total_time (global.h:55),
+ * for callers that advance it themselves (spilady.cpp:52): sld_check reports
+ * it and the barostats measure their intervals on it. sld_step advances the
+ * clock by itself. */
+int sld_set_total_time(sld_ctx* ctx, double total_time);
 /* The current box matrix of the selected replica (d.xx d.yx d.yy d.zx d.zy d.zz). */
 int sld_get_box(sld_ctx* ctx, double* box);
 /* The per-atom external field Hext (eV, i.e. -g muB B; external_field_CPU.cpp:

@@ -1521,6 +1521,29 @@ int sld_scale(sld_ctx* c, int step) {
     return read_flags(c);
 }
 
+int sld_scale_stress(sld_ctx* c) {
+    if (!c) return SLD_ERR_ARG;
+    if (!c->baro.stress_on) return set_err(c, SLD_ERR_STATE, "the STRESS barostat is off (sld_set_barostat)");
+    TRY(do_scale_stress(c));
+    drop_graphs(c);
+    return read_flags(c);
+}
+
+int sld_scale_pressure(sld_ctx* c) {
+    if (!c) return SLD_ERR_ARG;
+    if (!c->baro.pressure_on) return set_err(c, SLD_ERR_STATE, "the PRESSURE barostat is off (sld_set_barostat)");
+    TRY(do_scale_pressure(c));
+    drop_graphs(c);
+    return read_flags(c);
+}
+
+int sld_set_total_time(sld_ctx* c, double t) {
+    if (!c) return SLD_ERR_ARG;
+    if (!std::isfinite(t)) return set_err(c, SLD_ERR_ARG, "total_time is not finite");
+    c->total_time = t;
+    return SLD_OK;
+}
+
 int sld_get_box(sld_ctx* c, double* box) {
     if (!c || !box) return SLD_ERR_ARG;
     for (int k = 0; k < 6; ++k) box[k] = c->reps[c->sel].box.d[k];

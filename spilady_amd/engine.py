@@ -6,8 +6,11 @@ Engine does, and raises if the library is missing or no device is usable.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 from typing import Dict, List, Optional
 
 import numpy as np
@@ -21,6 +24,19 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "spilady_hip.h")
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _lib = None
+# every context not yet destroyed: the exit hook destroys them while the HIP
+# runtime is still whole (before interpreter teardown and the runtime's own
+# exit handlers), so no device call is left to a finaliser at shutdown
+_LIVE: "weakref.WeakSet[Engine]" = weakref.WeakSet()
+
+
+@atexit.register
+def _destroy_live_contexts():
+    for e in list(_LIVE):
+        try:
+            e.close()
+        except Exception:
+            pass
 
 ERRORS = {1: "SLD_ERR_ARG", 2: "SLD_ERR_HIP", 3: "SLD_ERR_CAPACITY", 4: "SLD_ERR_STATE",
           5: "SLD_ERR_NOGPU"}
@@ -141,6 +157,8 @@ class Engine(abi.EngineLike):
             rc = self.L.sld_create_batch(self.nrep, self._pv, self._av, int(compute_initial_forces),
                                          C.byref(h))
         self.h = h
+        if h:
+            _LIVE.add(self)
         if rc != 0:
             msg = self.L.sld_last_error(h).decode() if h else "sld_create failed"
             self.close()
@@ -164,11 +182,18 @@ class Engine(abi.EngineLike):
             raise SldError(rc, self.L.sld_last_error(self.h).decode())
 
     def close(self):
+        """sld_destroy (free_memory.cpp:31-52): every device allocation,
+        event, graph and the stream of the context."""
         if getattr(self, "h", None):
-            self.L.sld_destroy(self.h)
-            self.h = None
+            h, self.h = self.h, None
+            _LIVE.discard(self)
+            self.L.sld_destroy(h)
 
     def __del__(self):
+        # a context still open at interpreter shutdown was destroyed by the
+        # exit hook; a finaliser that runs later makes no device call
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -1,0 +1,77 @@
+"""SPILADY's own program on this library: the reference's main loop,
+initialize(), core(), check(), scale() and write() compiled from its sources
+with spilady_amd/csrc/spilady_hip_bridge.cpp in place of its CPU/CUDA
+backends (oracle/build_hiplib.sh, built where the reference's sources are
+present), with the switches of the shipped control.h: readconf, changestep
+with the time loop, inittime, Langevin spin and lattice thermostats, eltemp,
+renormalizeEnergy, localcolmot and writevsim.
+
+Started from the same con- file and variables.in, it must write exactly the
+files this project's own C++ driver (spilady_amd/spilady_hip) writes, byte
+for byte: the enr-/tmp-/spn-/str-/prs- logs, the con-/cel- configurations
+and the V_sim files."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import consistent_state, fe_params
+from spilady_amd import io
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "spilady_amd", "spilady_hip")
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "hiplib", "spilady")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="the SPILADY drop-in build is absent "
+                    "(oracle/build_hiplib.sh needs the reference's sources)")
+def test_spilady_main_on_the_library_writes_what_the_driver_writes(tmp_path):
+    v, p, at = fe_params(8, 300.0, 31, thermostats=True, eltemp=True, localcolmot=True, renormalize=True)
+    st = consistent_state(p, at)
+    state = io.AtomState(box=np.asarray(at.box), r=st["r"], p=st["p"], f=st["f"], s=st["s"],
+                         stress=st["stress"], rho=st["rho"], ke=st["ke"], pe=st["pe"], me=st["me"],
+                         me0=st["me0"])
+    # the shipped control.h: the time loop with changestep, every step printed
+    v = dict(v, in_config="con-start.dat", out_body="dropin", runstep=0, changestep=1,
+             total_production_time=5e-15, displace_limit=0.01, phi_limit=0.628319, interval_of_print_out=1,
+             interval_of_config_out=2, renormalizeEnergy=1, writevsim=1, interval_of_vsim=3, vsim_prec=6)
+    dirs = {}
+    for name in ("driver", "dropin"):
+        d = tmp_path / name
+        d.mkdir()
+        io.write_con(str(d / "con-start.dat"), state)
+        io.write_variables(str(d / "variables.in"), v)
+        dirs[name] = d
+    before = set(os.listdir(dirs["driver"]))
+    for name, exe, args in (("driver", DRIVER, ["variables.in"]), ("dropin", DROPIN, [])):
+        print(f"running {name}", flush=True)
+        res = subprocess.run([exe] + args, cwd=str(dirs[name]), capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, f"{name}: {res.stdout[-2000:]}\n{res.stderr[-2000:]}"
+    made = {n: set(os.listdir(d)) - before for n, d in dirs.items()}
+    logs = {f"{k}-dropin.dat" for k in ("enr", "tmp", "spn", "str", "prs")}
+    assert logs <= made["dropin"], sorted(made["dropin"])
+    assert any(f.startswith("con-dropin_") for f in made["dropin"])
+    assert any(f.startswith("cel-dropin_") for f in made["dropin"])
+    assert any(f.startswith("vsm-dropin_") for f in made["dropin"])
+    # every file both runs wrote, byte for byte (variables.all lists the
+    # reference's own variables, of which the driver echoes those it runs)
+    common = sorted((made["driver"] & made["dropin"]) - {"variables.all"})
+    assert logs <= set(common), (sorted(made["driver"]), sorted(made["dropin"]))
+    differ = []
+    for f in common:
+        a = (dirs["driver"] / f).read_bytes()
+        b = (dirs["dropin"] / f).read_bytes()
+        if a != b:
+            differ.append(f)
+    only = sorted(made["dropin"] ^ made["driver"])
+    assert not differ, f"files that differ: {differ}; written by one side only: {only}"
+    assert not [f for f in only if f != "variables.all"], only
+    # the time loop ran with a changing step
+    enr = io.read_log(str(dirs["dropin"] / "enr-dropin.dat"))
+    assert enr.shape[0] >= 5
+    assert len(set(np.diff(enr[:, 1]).round(25))) > 1
+    shutil.rmtree(str(tmp_path), ignore_errors=True)
