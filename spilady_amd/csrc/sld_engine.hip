@@ -105,7 +105,7 @@ struct sld_ctx {
     std::vector<size_t> alloc_bytes;  // size of each entry of allocs
     // launch shapes of the kernels whose results do not depend on them
     // (defaults: the autotuner's choice on the 2M-atom step on MI355X)
-    sld_shapes shapes{64, 64, 256, 2, 256};
+    sld_shapes shapes{64, 64, 256, 2, 256, 2048};
     // parametric potential model over the replicas' frozen configurations
     struct Model {
         bool ready = false;
@@ -434,6 +434,17 @@ int do_dTe(sld_ctx* c, double dt) {
     return SLD_OK;
 }
 
+// first level of a sweep's trailing run of levels that have at most `cap`
+// cells each, worth running in one workgroup (two levels or more, at most
+// kTailMaxLev); nlev when there is none or cap is 0
+int tail_start(const LevelSet& L, int cap) {
+    const int nl = L.nlev();
+    if (cap <= 0) return nl;
+    int lt = nl;
+    while (lt > 0 && nl - lt < kTailMaxLev && L.off[lt] - L.off[lt - 1] <= cap) --lt;
+    return nl - lt >= 2 ? lt : nl;
+}
+
 int do_dTe_launches(sld_ctx* c, double dt) {
     const int nloop = 50;
     double dtl = dt / nloop;
@@ -460,17 +471,37 @@ int do_dTe_launches(sld_ctx* c, double dt) {
             c->ev_span = true;
         }
         int nlaunch = 0;
+        bool reduced = false;
+        const Box& bx = c->reps[0].box;
+        const double ix2 = 1.0 / h0.dx2, iy2 = 1.0 / h0.dy2, iz2 = 1.0 / h0.dz2;
         for (int dir = 0; dir < 2; ++dir) {
             LevelSet& L = dir == 0 ? c->dte_fwd : c->dte_bwd;
-            for (int l = 0; l < L.nlev(); ++l) {
+            const int lt = tail_start(L, c->shapes.sweep_tail);
+            for (int l = 0; l < lt; ++l) {
                 int n = L.off[l + 1] - L.off[l];
                 double* part = dir == 1 ? c->d_part + (size_t)nb + npb : nullptr;
                 const int bs = c->shapes.sweep_block;
-                const Box& bx = c->reps[0].box;
-                LAUNCH(c, k_dte_sweep, nblk(n, bs), bs, L.d_cells + L.off[l], n, pref, 1.0 / h0.dx2,
-                       1.0 / h0.dy2, 1.0 / h0.dz2, bx.nx, bx.ny, bx.nz, c->D.Te, c->D.Ee, part);
+                LAUNCH(c, k_dte_sweep, nblk(n, bs), bs, L.d_cells + L.off[l], n, pref, ix2, iy2, iz2, bx.nx,
+                       bx.ny, bx.nz, c->D.Te, c->D.Ee, part);
                 if (dir == 1) npb += nblk(n, 64);  // one partial per wave
                 nlaunch += n > 0;
+            }
+            if (lt < L.nlev()) {
+                // the small trailing levels in one workgroup; after the
+                // backward sweep's, the same block adds the sub-step's totals
+                SweepTail T{};
+                T.nlev = L.nlev() - lt;
+                for (int l = lt; l <= L.nlev(); ++l) T.off[l - lt] = L.off[l];
+                for (int l = lt; l < L.nlev(); ++l) {
+                    T.poff[l - lt] = npb;
+                    if (dir == 1) npb += nblk(L.off[l + 1] - L.off[l], 64);
+                }
+                double* part = dir == 1 ? c->d_part + (size_t)nb : nullptr;
+                LAUNCH(c, k_dte_tail, 1, 1024, L.d_cells, T, pref, ix2, iy2, iz2, bx.nx, bx.ny, bx.nz, c->D.Te,
+                       c->D.Ee, part, (const double*)c->d_part, nb, (const double*)(c->d_part + nb),
+                       dir == 1 ? npb : 0, dir == 1 ? c->d_scal + 8 : nullptr);
+                reduced = dir == 1;
+                nlaunch += 1;
             }
         }
         if (ev) {
@@ -478,8 +509,9 @@ int do_dTe_launches(sld_ctx* c, double dt) {
             (void)hipEventRecord(ev[1], c->stream);
             c->ev_launches[(ev - c->ev_pool.data()) / 2] = nlaunch;
         }
-        LAUNCH(c, k_reduce2, 1, 1024, (const double*)c->d_part, nb, (const double*)(c->d_part + nb), npb,
-               c->d_scal + 8);
+        if (!reduced)
+            LAUNCH(c, k_reduce2, 1, 1024, (const double*)c->d_part, nb, (const double*)(c->d_part + nb), npb,
+                   c->d_scal + 8);
         if (it + 1 < nloop)
             LAUNCH(c, k_dte_finish_ab, nb, 256, h, M, (const double*)(c->d_scal + 8), 1, c->d_part);
         else
@@ -1565,8 +1597,12 @@ int sld_set_external(sld_ctx* c, const double* hext, const double* fext) {
 }  // extern "C"
 
 namespace {
-const int kShapeCand[5][3] = {{32, 64, 128}, {64, 128, 256}, {64, 128, 256}, {1, 2, 4}, {64, 128, 256}};
-const char* kShapeKernel[5] = {"k_spin_phase", "k_dte_sweep", "k_force", "k_nbuild", "k_temperature"};
+constexpr int kNShape = 6;
+const int kShapeCand[kNShape][3] = {{32, 64, 128}, {64, 128, 256}, {64, 128, 256}, {1, 2, 4}, {64, 128, 256},
+                                    {0, 2048, 8192}};
+// the kernel each shape is timed on (the sweep span covers the tail launches)
+const char* kShapeKernel[kNShape] = {"k_spin_phase", "k_dte_sweep", "k_force", "k_nbuild", "k_temperature",
+                                     "k_dte_sweep"};
 
 int& shape_field(sld_shapes& s, int k) {
     switch (k) {
@@ -1574,13 +1610,14 @@ int& shape_field(sld_shapes& s, int k) {
         case 1: return s.sweep_block;
         case 2: return s.force_block;
         case 3: return s.nbuild_cells;
-        default: return s.temperature_block;
+        case 4: return s.temperature_block;
+        default: return s.sweep_tail;
     }
 }
 
 bool shape_valid(const sld_shapes& s) {
     sld_shapes t = s;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < kNShape; ++k) {
         const int v = shape_field(t, k);
         if (v != kShapeCand[k][0] && v != kShapeCand[k][1] && v != kShapeCand[k][2]) return false;
     }
@@ -1658,7 +1695,7 @@ int time_in_steps(sld_ctx* c, const char* name, int steps, double* us, int* laun
 
 int autotune_body(sld_ctx* c, int steps, double* us) {
     TRY(do_core(c));  // warm-up
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < kNShape; ++k) {
         double best = 0.0;
         int best_v = shape_field(c->shapes, k);
         for (int q = 0; q < 3; ++q) {

@@ -108,6 +108,18 @@ __global__ void k_dte_ab(double h, Mesh M, double* partials);
 __global__ void k_dte_sweep(const int* cells, int ncell, double pref, double idx2, double idy2,
                             double idz2, int nx, int ny, int nz, double* Te, double* Ee, double* partials);
 __global__ void k_reduce2(const double* pa, int na, const double* pb, int nb, double* out);
+// the trailing small levels of a C sweep, run by one workgroup: level l of
+// the run holds entries [off[l], off[l + 1]) of the sweep's level list, and
+// its backward-sweep partials start at poff[l]
+constexpr int kTailMaxLev = 32;
+struct SweepTail {
+    int nlev;
+    int off[kTailMaxLev + 1];
+    int poff[kTailMaxLev];
+};
+__global__ void k_dte_tail(const int* cells, SweepTail T, double pref, double idx2, double idy2, double idz2,
+                           int nx, int ny, int nz, double* Te, double* Ee, double* partials, const double* pa,
+                           int na, const double* pb, int nb, double* out);
 __global__ void k_dte_finish(double h, Mesh M, const double* xp, int nxp);
 __global__ void k_dte_finish_ab(double h, Mesh M, const double* xp, int nxp, double* partials);
 __global__ void k_obs_atoms(Tables T, Phys P, Dev D, double* partials, int scan_spin);

@@ -1139,43 +1139,42 @@ __global__ __launch_bounds__(256) void k_dte_ab(double h, Mesh M, double* partia
 // In the backward sweep every cell is updated exactly once, last, so the
 // launch also leaves partials of the final energies (partials != null), one
 // per wave of 64 level entries: the sums do not depend on the block size.
+// the update of one cell of a C-sweep level, given as packed coordinates
+// ix | iy << 10 | iz << 20; returns its new Ee
+__device__ __forceinline__ double dte_cell_update(int pk, double pref, double idx2, double idy2, double idz2, int nx,
+                                                  int ny, int nz, double* Te, double* Ee) {
+    const int sl = nx * ny;
+    const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = pk >> 20;
+    const int c = ix + iy * nx + iz * sl;
+    const int xm = ix == 0 ? nx - 1 : ix - 1, xp = ix + 1 == nx ? 0 : ix + 1;
+    const int ym = iy == 0 ? ny - 1 : iy - 1, yp = iy + 1 == ny ? 0 : iy + 1;
+    const int zm = iz == 0 ? nz - 1 : iz - 1, zp = iz + 1 == nz ? 0 : iz + 1;
+    const int row = iy * nx + iz * sl;
+    const double tc = Te[c];
+    const double txm = Te[xm + row], txp = Te[xp + row];
+    const double tym = Te[ix + ym * nx + iz * sl], typ = Te[ix + yp * nx + iz * sl];
+    const double tzm = Te[ix + iy * nx + zm * sl], tzp = Te[ix + iy * nx + zp * sl];
+    double ee = Ee[c];
+    ee += pref * ((txp + txm - 2.0 * tc) * idx2 + (typ + tym - 2.0 * tc) * idy2 + (tzp + tzm - 2.0 * tc) * idz2);
+    Ee[c] = ee;
+    Te[c] = ee_to_te(ee);
+    return ee;
+}
+
 __global__ __launch_bounds__(256) void k_dte_sweep(const int* cells, int ncell, double pref, double idx2,
                                                    double idy2, double idz2, int nx, int ny, int nz, double* Te,
                                                    double* Ee, double* partials) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
-    if (t < ncell) {
-        const int pk = cells[t];  // ix | iy << 10 | iz << 20
-        const int sl = nx * ny;
-        const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = pk >> 20;
-        const int c = ix + iy * nx + iz * sl;
-        const int xm = ix == 0 ? nx - 1 : ix - 1, xp = ix + 1 == nx ? 0 : ix + 1;
-        const int ym = iy == 0 ? ny - 1 : iy - 1, yp = iy + 1 == ny ? 0 : iy + 1;
-        const int zm = iz == 0 ? nz - 1 : iz - 1, zp = iz + 1 == nz ? 0 : iz + 1;
-        const int row = iy * nx + iz * sl;
-        const double tc = Te[c];
-        const double txm = Te[xm + row], txp = Te[xp + row];
-        const double tym = Te[ix + ym * nx + iz * sl], typ = Te[ix + yp * nx + iz * sl];
-        const double tzm = Te[ix + iy * nx + zm * sl], tzp = Te[ix + iy * nx + zp * sl];
-        double ee = Ee[c];
-        ee += pref * ((txp + txm - 2.0 * tc) * idx2 + (typ + tym - 2.0 * tc) * idy2 + (tzp + tzm - 2.0 * tc) * idz2);
-        Ee[c] = ee;
-        Te[c] = ee_to_te(ee);
-        acc[0] = ee;
-    }
+    if (t < ncell) acc[0] = dte_cell_update(cells[t], pref, idx2, idy2, idz2, nx, ny, nz, Te, Ee);
     if (partials) {
         const double v = wave_sum(acc[0]);
         if ((threadIdx.x & 63) == 0) partials[t >> 6] = v;
     }
 }
 
-// the two totals of a sub-step: out[0] = sum of the starting-energy partials,
-// out[1] = sum of the final-energy partials; one block of 1024 threads in a
-// fixed order, independent of how the partials were produced. Each thread
-// keeps eight loads of each array in flight, a wave sums by shuffles and
-// thread 0 adds the 16 wave sums in order.
-__global__ __launch_bounds__(1024) void k_reduce2(const double* pa, int na, const double* pb, int nb,
-                                                  double* out) {
+// the two totals of a sub-step (see k_reduce2) by one block of 1024 threads
+__device__ __forceinline__ void reduce2_block(const double* pa, int na, const double* pb, int nb, double* out) {
     __shared__ double sh[2][16];
     const int t = threadIdx.x;
     double a = 0.0, e = 0.0;
@@ -1209,6 +1208,45 @@ __global__ __launch_bounds__(1024) void k_reduce2(const double* pa, int na, cons
         out[0] = A;
         out[1] = E;
     }
+}
+
+// The trailing small levels of a C sweep in one workgroup of 1024 threads:
+// level after level, separated by workgroup barriers (all of the block's
+// waves share one compute unit and its write-through L1, so a barrier makes
+// one level's stores visible to the next level's loads). Each cell update
+// and each backward-sweep partial (one per 64 entries of a level, at
+// T.poff[level] + entry / 64) is exactly what a k_dte_sweep launch per level
+// would produce. With out != null the block then adds the sub-step's two
+// totals from pa[0, na) and pb[0, nb), as k_reduce2 does.
+__global__ __launch_bounds__(1024) void k_dte_tail(const int* cells, SweepTail T, double pref, double idx2,
+                                                   double idy2, double idz2, int nx, int ny, int nz, double* Te,
+                                                   double* Ee, double* partials, const double* pa, int na,
+                                                   const double* pb, int nb, double* out) {
+    const int t = threadIdx.x;
+    for (int l = 0; l < T.nlev; ++l) {
+        const int o = T.off[l], n = T.off[l + 1] - o;
+        for (int e0 = 0; e0 < n; e0 += 1024) {
+            const int e = e0 + t;
+            double acc = 0.0;
+            if (e < n) acc = dte_cell_update(cells[o + e], pref, idx2, idy2, idz2, nx, ny, nz, Te, Ee);
+            if (partials) {
+                const double v = wave_sum(acc);
+                if ((t & 63) == 0 && e < n) partials[T.poff[l] + (e >> 6)] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (out) reduce2_block(pa, na, pb, nb, out);
+}
+
+// the two totals of a sub-step: out[0] = sum of the starting-energy partials,
+// out[1] = sum of the final-energy partials; one block of 1024 threads in a
+// fixed order, independent of how the partials were produced. Each thread
+// keeps eight loads of each array in flight, a wave sums by shuffles and
+// thread 0 adds the 16 wave sums in order.
+__global__ __launch_bounds__(1024) void k_reduce2(const double* pa, int na, const double* pb, int nb,
+                                                  double* out) {
+    reduce2_block(pa, na, pb, nb, out);
 }
 
 // starting and final totals of a sub-step from n partial pairs (xp[0..n):

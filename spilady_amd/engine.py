@@ -262,7 +262,7 @@ class Engine(abi.EngineLike):
         `steps` steps per candidate; the state is restored afterwards. Returns
         (chosen shapes, {kernel field: {candidate: device us per step}})."""
         sh = abi.SldShapes()
-        us = np.zeros(15)
+        us = np.zeros(3 * len(abi.SldShapes._fields_))
         self._rc(self.L.sld_autotune(self.h, int(steps), C.byref(sh), us.ctypes.data_as(_dp)))
         table = {}
         for k, (name, _) in enumerate(abi.SldShapes._fields_):

@@ -14,9 +14,11 @@ from spilady_amd import abi
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0", "stress", "heff_h", "heff_l")
+# sweep_tail 0 launches every diffusion level on its own; the default (2048)
+# and 8192 run the trailing levels of each sweep in one workgroup
 OTHER_SHAPES = [
-    dict(spin_block=32, sweep_block=256, force_block=128, nbuild_cells=1, temperature_block=64),
-    dict(spin_block=128, sweep_block=128, force_block=64, nbuild_cells=4, temperature_block=128),
+    dict(spin_block=32, sweep_block=256, force_block=128, nbuild_cells=1, temperature_block=64, sweep_tail=0),
+    dict(spin_block=128, sweep_block=128, force_block=64, nbuild_cells=4, temperature_block=128, sweep_tail=8192),
 ]
 
 
