@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -176,30 +177,25 @@ def batched_vs_serial(steps: int, warmup: int):
                         "seed 100+r, Langevin lattice+spin, dt 1 fs"}
 
 
-def cpu_baseline(v, st_dict, box, natom, steps: int):
-    """The compiled reference (oracle/_ref) on the same atoms, timed on the
-    host's cores for a bounded number of steps; falls back to the
-    single-threaded plain-C oracle when the reference library is absent."""
+def cpu_baseline(v, st_dict, box, natom, steps: int, n_unit: int, repeats: int = 3):
+    """The compiled reference (oracle/_ref) on the same workload, timed on the
+    host's cores in a child process (oracle/cpu_baseline.py) whose OpenMP
+    runtime starts bound one thread per physical core of the lease: one
+    warm-up step, then the median of `repeats` timings of `steps` steps.
+    Falls back to the single-threaded plain-C oracle in this process when
+    the reference library is absent."""
     from oracle import refpy
-    threads = int(os.environ.get("SLD_CPU_THREADS", min(16, os.cpu_count() or 1)))
-    vv = dict(v)
-    vv["OMP_threads"] = threads
     if refpy.available():
-        from spilady_amd.io import AtomState
-        st = AtomState(box=np.asarray(box), r=st_dict["r"], p=st_dict["p"], f=st_dict["f"],
-                       s=st_dict["s"], stress=st_dict["stress"], rho=st_dict["rho"],
-                       ke=st_dict["ke"], pe=st_dict["pe"], me=st_dict["me"], me0=st_dict["me0"])
-        d = tempfile.mkdtemp(prefix="sldref_bench_")
-        ref = refpy.Reference()
-        ref.init_memory(d, vv, st)
-        t0 = time.perf_counter()
-        ref.run(0, steps, False)
-        dt = time.perf_counter() - t0
-        ref.free()
-        return {"value": natom * steps / dt, "unit": "atom-steps/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"{steps} full SLDHL steps of the same {natom}-atom workload "
-                          f"(compiled reference, OpenMP {threads} threads), {dt:.1f} s"}
+        from oracle import cpu_baseline as cb
+        threads = int(os.environ.get("SLD_CPU_THREADS", "0")) or cb.default_threads()
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
+                   OMP_SCHEDULE="static")
+        out = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n-unit", str(n_unit),
+                              "--threads", str(threads), "--steps", str(steps), "--repeats", str(repeats)],
+                             cwd=ROOT, env=env, capture_output=True, text=True, timeout=1800)
+        if out.returncode != 0:
+            raise RuntimeError(f"cpu baseline failed: {out.stderr[-2000:]}")
+        return json.loads(out.stdout.strip().splitlines()[-1])
     from oracle import orcpy
     from spilady_amd import abi
     p = abi.params_from_variables(v, box, natom)
@@ -220,7 +216,8 @@ def main() -> int:
     ap.add_argument("--n-unit", type=int, default=100, help="cubic cells per side (100 = 2M atoms)")
     ap.add_argument("--kernels", default=",".join(KERNELS),
                     help="kernels timed with HIP events in extra steps after the timed region")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps per timed repeat of the CPU baseline")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="timed repeats of the CPU baseline (median)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nve-drift", type=int, default=0,
                     help="instead of the benchmark, measure NVE energy drift over this many steps")
@@ -322,7 +319,7 @@ def main() -> int:
     adjoint = adjoint_leg(args.adjoint_reps) if args.adjoint_reps > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(v, st, at.box, n, args.cpu_steps)
+        cpu = cpu_baseline(v, st, at.box, n, args.cpu_steps, args.n_unit, args.cpu_repeats)
     out = {
         "metric": METRIC,
         "value": value,
