@@ -43,6 +43,7 @@ PEAK_HBM_GBS = 8000.0
 # 14 within rcut_mag per atom.
 PAIRS_POT = 26
 PAIRS_MAG = 14
+NLOOP_DTE = 50          # core_dTe sub-steps per call (core_dTe_CPU.cpp:51)
 KERNELS = {
     # one cell update of the diffusion sweep: 3 x (2 add, 1 mul, 1 sub, 1 div)
     # + 3 add + 1 mul, then T(E) = acosh(exp(bE/3))/b (4); reads Te, Ee of the
@@ -62,7 +63,6 @@ KERNELS = {
 # eltemp (electron cell state spread over 6.09 atoms per cell)
 STEP_FLOP = 14.6e3
 STEP_BYTES = 381.0
-NLOOP_DTE = 50
 
 
 def traffic_of(kname: str):
@@ -129,6 +129,72 @@ def batch_workload(nrep: int = 64, seed: int = 3):
     from spilady_amd import abi, configs
     return [(abi.params_from_variables(v, at.box, at.natom, replica=r), at)
             for r, (v, at) in enumerate(configs.batch_replicas(nrep, seed))]
+
+
+def adjoint_leg(reps: int):
+    """Loss gradient of the potential-fitting workload: 32 frozen bcc Fe
+    configurations of 16^3 cubic cells (262,144 atoms), MSE of energies,
+    forces and fields against targets at 2%-perturbed parameters. One
+    gradient = sld_model_eval + sld_model_vjp on device buffers (the table
+    build of sld_model_set_params runs once per parameter vector and is
+    reported apart)."""
+    import torch
+    from spilady_amd import fit
+    confs = fit.frozen_configs()
+    natoms = np.array([c.r.shape[0] for c in confs], dtype=np.float64)
+    m = fit.PotentialModel(confs)
+    th0 = fit.default_params()
+    t0 = time.perf_counter()
+    m.set_params(fit.perturbed_params(th0, 0.02, 999))
+    t_set = time.perf_counter() - t0
+    dev = torch.device("cuda")
+    targets = tuple(torch.tensor(x, device=dev) for x in m.eval())
+    n_t = torch.tensor(natoms, device=dev)
+    m.set_params(th0)
+    E = torch.empty(m.nconf, dtype=torch.float64, device=dev)
+    F = torch.empty((m.ntot, 3), dtype=torch.float64, device=dev)
+    H = torch.empty_like(F)
+    gE = torch.empty_like(E)
+    gF = torch.empty_like(F)
+    gH = torch.empty_like(F)
+    grad = torch.empty(fit.NPARAM, dtype=torch.float64, device=dev)
+    L, h = m.L, m.h
+
+    def fwd():
+        m.engine._rc(L.sld_model_eval(h, E.data_ptr(), F.data_ptr(), H.data_ptr()))
+
+    def one():
+        fwd()
+        Et, Ft, Ht = targets
+        gE.copy_(2.0 * (E - Et) / (n_t * n_t) / E.numel())
+        gF.copy_(2.0 * (F - Ft) / F.numel())
+        gH.copy_(2.0 * (H - Ht) / H.numel())
+        torch.cuda.synchronize()
+        m.engine._rc(L.sld_model_vjp(h, gE.data_ptr(), gF.data_ptr(), gH.data_ptr(), grad.data_ptr()))
+
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    kern = {}
+    # device time of each kernel from HIP events on the library's stream: the
+    # timing covers one entry-point call (k_model_rho runs in both)
+    for kname, call in (("k_model_rho", fwd), ("k_model_forward", fwd), ("k_model_q", one),
+                        ("k_model_adjoint", one)):
+        m.engine.time_kernel(kname)
+        call()
+        launches, kms = m.engine.kernel_timing()
+        m.engine.time_kernel(None)
+        kern[kname] = kms / max(launches, 1) * 1e3
+    m.close()
+    ntot = int(natoms.sum())
+    return {"workload": "32 frozen bcc Fe configurations of 16^3 cubic cells (262,144 atoms), sigma 0.08 A, "
+                        "spin tilts <= 30 deg, seeds 200-231; MSE loss of E, F, H",
+            "atoms": ntot, "ms_per_loss_gradient": wall * 1e3, "atoms_per_s": ntot / wall,
+            "set_params_s": t_set, "kernel_us": kern}
 
 
 def batched_vs_serial(steps: int, warmup: int):

@@ -269,6 +269,8 @@ typedef struct {
     int sweep_tail;        /* core_dTe diffusion: the trailing levels of each sweep with at most this many
                               cells run in one workgroup, level after level (0: every level its own
                               launch); 0, 2048 or 8192 */
+    int spin_lanes;        /* core_ds sweep level: lanes that share one cell's walk (the neighbour
+                              gathers and field sums split among them); 1, 2 or 4 */
 } sld_shapes;
 int sld_get_shapes(sld_ctx* ctx, sld_shapes* out);
 /* How sld_step issues its work (no counterpart in the reference): 0 (default)
@@ -282,7 +284,7 @@ int sld_set_shapes(sld_ctx* ctx, const sld_shapes* in);
  * one warm-up step) with HIP events, keeps the fastest, and restores the
  * context's state afterwards, so the trajectory continues exactly as if the
  * call had not been made. best (may be NULL) receives the chosen shapes;
- * us (may be NULL, 18 entries: kernel-major in the order of sld_shapes, three
+ * us (may be NULL, 21 entries: kernel-major in the order of sld_shapes, three
  * candidates each in ascending order) the device microseconds per step of
  * each candidate, 0 where the kernel did not run. Needs device memory for a
  * copy of the context's state while it runs. */

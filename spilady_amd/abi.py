@@ -72,12 +72,13 @@ class SldCells(C.Structure):
 
 class SldShapes(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("spin_block", "sweep_block", "force_block", "nbuild_cells",
-                                         "temperature_block", "sweep_tail")]
+                                         "temperature_block", "sweep_tail", "spin_lanes")]
 
 
 SHAPE_CANDIDATES = {"spin_block": (32, 64, 128), "sweep_block": (64, 128, 256),
                     "force_block": (64, 128, 256), "nbuild_cells": (1, 2, 4),
-                    "temperature_block": (64, 128, 256), "sweep_tail": (0, 2048, 8192)}
+                    "temperature_block": (64, 128, 256), "sweep_tail": (0, 2048, 8192),
+                    "spin_lanes": (1, 2, 4)}
 
 
 class SldBarostat(C.Structure):

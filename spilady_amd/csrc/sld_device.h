@@ -7,6 +7,14 @@
 #include "heatcap_poly.h"
 #include "sld_common.h"
 
+// Floating-point contraction is decided by the source alone: a multiply and
+// an add fuse only inside one expression, never across statements by the
+// backend. One device function inlined into two kernels (or two template
+// instances of one kernel) then rounds identically in both, which the
+// bitwise equalities between launch shapes, batched and single contexts, and
+// graph and eager launches rest on.
+#pragma clang fp contract(on)
+
 namespace sld {
 
 // ---------------------------------------------------------------------------

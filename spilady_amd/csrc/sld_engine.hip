@@ -105,7 +105,7 @@ struct sld_ctx {
     std::vector<size_t> alloc_bytes;  // size of each entry of allocs
     // launch shapes of the kernels whose results do not depend on them
     // (defaults: the autotuner's choice on the 2M-atom step on MI355X)
-    sld_shapes shapes{64, 64, 256, 2, 256, 2048};
+    sld_shapes shapes{64, 64, 256, 2, 256, 2048, 4};
     // parametric potential model over the replicas' frozen configurations
     struct Model {
         bool ready = false;
@@ -156,7 +156,10 @@ inline int xcd_grid(int nlogical) { return (nlogical + 7) / 8 * 8; }
 
 // Event pair around a launch of the timed kernel (see sld_time_kernel).
 hipEvent_t* timing_slot(sld_ctx* c, const char* name) {
-    if (c->ev_span || c->timed_name.empty() || c->timed_name != name) return nullptr;
+    if (c->ev_span || c->timed_name.empty()) return nullptr;
+    // a template instance (k_spin_phase<4>) is timed under its kernel's name
+    const size_t ln = std::strcspn(name, "<");
+    if (c->timed_name.size() != ln || c->timed_name.compare(0, ln, name, ln) != 0) return nullptr;
     if (c->ev_used + 2 > (int)c->ev_pool.size()) {
         if (c->ev_pool.size() >= 8192) return nullptr;
         for (int k = 0; k < 64; ++k) {
@@ -359,10 +362,16 @@ int do_ds(sld_ctx* c, double dt) {
         if (c->phys.spinlang)
             LAUNCH(c, k_spin_noise, nblk(c->D.N, 256), 256, c->D, (int)(c->pass - c->pass_dev));
         for (int l = 0; l < L.nlev(); ++l) {
-            int n = L.off[l + 1] - L.off[l];
-            const int bs = c->shapes.spin_block;
-            LAUNCH(c, k_spin_phase, nblk(n, bs), bs, L.d_cells + L.off[l], n, dir, dth, c->tab,
-                   c->phys, c->D);
+            const int n = L.off[l + 1] - L.off[l];
+            const int bs = c->shapes.spin_block, lpc = c->shapes.spin_lanes;
+            const int* cells = L.d_cells + L.off[l];
+            // only the backward sweep's Heff_H, Heff_L and sum J|s_j| last
+            if (lpc == 4)
+                LAUNCH(c, k_spin_phase<4>, nblk(n * 4, bs), bs, cells, n, dir, dir, dth, c->tab, c->phys, c->D);
+            else if (lpc == 2)
+                LAUNCH(c, k_spin_phase<2>, nblk(n * 2, bs), bs, cells, n, dir, dir, dth, c->tab, c->phys, c->D);
+            else
+                LAUNCH(c, k_spin_phase<1>, nblk(n, bs), bs, cells, n, dir, dir, dth, c->tab, c->phys, c->D);
         }
         c->pass++;
     }
@@ -834,6 +843,12 @@ int up4(sld_ctx* c, double4* d, const double* src, int r, bool with_len) {
     return SLD_OK;
 }
 
+// electron energy of a cell at Te (eV), in the arithmetic of heatcapacity_CPU.cpp:42-48
+double host_te_to_ee(double te) {
+    const double bb = 2e-4 / kBoltz;
+    return 3e0 / bb * std::log(std::cosh(bb * te));
+}
+
 int up1(sld_ctx* c, double* d, const double* src, size_t off, size_t n) {
     if (!src) return SLD_OK;
     HIPCHK(c, hipMemcpyAsync(d + off, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1222,8 +1237,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     // electron temperatures (map_cells.cpp:128-159)
     if (P0.eltemp) {
         std::vector<double> te(C, P0.temperature), ee(C);
-        const double bb = 2e-4 / kBoltz;
-        for (int k = 0; k < C; ++k) ee[k] = 3e0 / bb * std::log(std::cosh(bb * te[k]));
+        for (int k = 0; k < C; ++k) ee[k] = host_te_to_ee(te[k]);
         TRY(up1(c, D.Te, te.data(), 0, C));
         TRY(up1(c, D.Ee, ee.data(), 0, C));
     }
@@ -1345,8 +1359,10 @@ int sld_core_dr(sld_ctx* c, double dt) { if (!c) return SLD_ERR_ARG; TRY(do_dr(c
 
 int sld_core_dTe(sld_ctx* c, double dt) {
     if (!c->phys.eltemp) return set_err(c, SLD_ERR_STATE, "core_dTe needs eltemp");
+    c->ev_used = 0;
     TRY(do_dTe(c, dt));
-    return read_flags(c);
+    TRY(read_flags(c));
+    return collect_timing(c);
 }
 
 int sld_calculate_temperature(sld_ctx* c) {
@@ -1489,9 +1505,11 @@ int sld_get_cells(sld_ctx* c, sld_cells* o) {
 int sld_set_cells_te(sld_ctx* c, const double* te) {
     if (!c || !te) return SLD_ERR_ARG;
     const Rep& R = c->reps[c->sel];
+    // Ee = E(Te) on the host, as map_cells.cpp:154 and sld_create compute it
+    std::vector<double> ee(R.ncell);
+    for (int k = 0; k < R.ncell; ++k) ee[k] = host_te_to_ee(te[k]);
     TRY(up1(c, c->D.Te, te, R.cell0, R.ncell));
-    LAUNCH(c, k_cells_ee, nblk(R.ncell, 256), 256, c->D, R.cell0, R.ncell);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    TRY(up1(c, c->D.Ee, ee.data(), R.cell0, R.ncell));
     return SLD_OK;
 }
 
@@ -1597,12 +1615,12 @@ int sld_set_external(sld_ctx* c, const double* hext, const double* fext) {
 }  // extern "C"
 
 namespace {
-constexpr int kNShape = 6;
+constexpr int kNShape = 7;
 const int kShapeCand[kNShape][3] = {{32, 64, 128}, {64, 128, 256}, {64, 128, 256}, {1, 2, 4}, {64, 128, 256},
-                                    {0, 2048, 8192}};
+                                    {0, 2048, 8192}, {1, 2, 4}};
 // the kernel each shape is timed on (the sweep span covers the tail launches)
 const char* kShapeKernel[kNShape] = {"k_spin_phase", "k_dte_sweep", "k_force", "k_nbuild", "k_temperature",
-                                     "k_dte_sweep"};
+                                     "k_dte_sweep", "k_spin_phase"};
 
 int& shape_field(sld_shapes& s, int k) {
     switch (k) {
@@ -1611,7 +1629,8 @@ int& shape_field(sld_shapes& s, int k) {
         case 2: return s.force_block;
         case 3: return s.nbuild_cells;
         case 4: return s.temperature_block;
-        default: return s.sweep_tail;
+        case 5: return s.sweep_tail;
+        default: return s.spin_lanes;
     }
 }
 
@@ -1743,7 +1762,7 @@ int sld_autotune(sld_ctx* c, int steps, sld_shapes* best, double* us) {
     if (!c) return SLD_ERR_ARG;
     if (steps < 1) return set_err(c, SLD_ERR_ARG, "autotune needs at least one timed step");
     if (us)
-        for (int k = 0; k < 15; ++k) us[k] = 0.0;
+        for (int k = 0; k < 3 * kNShape; ++k) us[k] = 0.0;
     const sld_shapes before = c->shapes;
     Snapshot snap;
     TRY(snapshot_take(c, snap));
@@ -1899,3 +1918,4 @@ int sld_model_natoms(const sld_ctx* c) {
 }
 
 }  // extern "C"
+

@@ -92,8 +92,11 @@ __global__ void k_reduce_max2(const double* partials, int nblk, double* out);
 __global__ void k_pass_advance(Dev D, int n);
 __global__ void k_qn_init(Phys P, Dev D);
 __global__ void k_qn_refresh(Phys P, Dev D, uint64_t r);
-__global__ void k_spin_phase(const int* cells, int ncell, int backward, double dt, Tables T, Phys P,
-                             Dev D);
+// LPC = 1, 2 or 4 lanes per cell; store_fields: write Heff_H, Heff_L and
+// sum J|s_j| (the backward sweep of a core_ds call)
+template <int LPC>
+__global__ void k_spin_phase(const int* cells, int ncell, int backward, int store_fields, double dt, Tables T,
+                             Phys P, Dev D);
 __global__ void k_dp_cells(double dt, Phys P, Dev D, int pass_off, double* partials);
 __global__ void k_dp_center(Phys P, Dev D, const double* scal, double* partials);
 __global__ void k_dp_scale(Phys P, Dev D, const double* scal);
@@ -128,7 +131,6 @@ __global__ void k_renorm(Dev D, const double* err, double* partials);
 __global__ void k_reduce(const double* partials, int nblk, int nk, double* out);
 __global__ void k_reduce_seg(const double* partials, const int* seg, int nk, double* out, int out_off);
 __global__ void k_check_nonfinite(Dev D);
-__global__ void k_cells_ee(Dev D, int c0, int n);
 __global__ void k_spin_len(Dev D, int a0, int n);
 
 // ---- parametric potential model (sld_model.hip) ----

@@ -553,43 +553,39 @@ __device__ __forceinline__ void exchange_field(int a, const Dev& D, const Phys& 
 }
 
 // One dependency level of a core_ds sweep: each listed cell walks its list
-// head to tail (forward) or tail to head (backward), one thread per cell
+// head to tail (forward) or tail to head (backward)
 // (calculate_spin_CPU.cpp:33-235, core_ds_CPU.cpp:35-83).
 //
-// A level holds a few thousand cells, so each thread's chain of dependent
-// work is the level's duration. Everything that does not depend on other
-// atoms' new spins is taken off that chain:
+// A level holds a few thousand to forty thousand cells, and each cell's walk
+// is a chain of dependent work, so the level lasts as long as one chain.
+// Everything that does not depend on other atoms' new spins is taken off that
+// chain:
 // - the Langevin normals of the pass come from k_spin_noise, a fully
 //   parallel kernel launched before the sweep;
 // - the per-atom work is software pipelined: during the update of atom q the
-//   thread already holds atom q+1's neighbour ids and J values, has issued
+//   lanes already hold atom q+1's neighbour ids and J values, have issued
 //   the loads of those neighbours' spins and of atom q+1's own spin, noise
-//   and Landau coefficients, and has atom q+2's ids and density in flight.
+//   and Landau coefficients, and have atom q+2's ids and density in flight.
 // The only spin such a load can miss is atom q's own new value (written after
 // the load was issued). The field is summed over the loaded spins and then
 // corrected by J_q (s_q,new - s_q,old), where J_q is atom q's weight in the
 // list (zero if it is not a neighbour), so every atom sees the spins the
 // sequential sweep shows it, up to the rounding of that correction.
+//
+// The field sums H = sum J_e s_e and S = sum J_e |s_e| run over the
+// atom's rcut_mag list as four partial sums: P_r adds the entries 4r..4r+3,
+// then 16+4r..16+4r+3, in order, and H = Hext + ((P0 + P1) + (P2 + P3)).
+// A cell's walk is spread over LPC = 1, 2 or 4 lanes: lane j of the cell
+// holds the partials r = j, j + LPC, ..., gathers their entries, and the
+// lanes combine them by shuffles in that same order; every lane then runs the
+// per-atom update (the same arithmetic on the same values, so the same bits)
+// and lane 0 stores the result. More lanes per cell put more waves on a level
+// (a 39,304-cell level fills 614 waves with one lane per cell, 2,456 with four)
+// and keep more neighbour loads in flight per atom; the results are bitwise
+// the same for every LPC.
 
-constexpr int kPipe = 16;   // rcut_mag neighbours held in registers (more: loaded directly)
-
-__device__ __forceinline__ void load_ids(const Dev& D, int a, int (&id)[kPipe]) {
-    const int4* p = reinterpret_cast<const int4*>(D.nbm + (size_t)a * kMaxMag);
-#pragma unroll
-    for (int q = 0; q < kPipe / 4; ++q) {
-        int4 v = p[q];
-        id[4 * q] = v.x; id[4 * q + 1] = v.y; id[4 * q + 2] = v.z; id[4 * q + 3] = v.w;
-    }
-}
-
-__device__ __forceinline__ void load_js(const Dev& D, int a, double (&J)[kPipe]) {
-    const double2* p = reinterpret_cast<const double2*>(D.nbJ + (size_t)a * kMaxMag);
-#pragma unroll
-    for (int q = 0; q < kPipe / 2; ++q) {
-        double2 v = p[q];
-        J[2 * q] = v.x; J[2 * q + 1] = v.y;
-    }
-}
+constexpr int kPipe = 16;      // rcut_mag neighbours loaded ahead (more: loaded when used)
+constexpr int kMagPart = 4;    // partial sums of the field, 4 entries per chunk
 
 // rotation of s about Heff by |Heff| dt / hbar (core_ds_CPU.cpp:90-127); the
 // axis and angle are computed once for both half-rotations of a step
@@ -669,9 +665,13 @@ __device__ __forceinline__ LanRows landau_rows(const Tables& T, double rho) {
     return L;
 }
 
-__global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell, int backward, double dt,
-                                                    Tables T, Phys P, Dev D) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+template <int LPC>
+__global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell, int backward, int store_fields,
+                                                    double dt, Tables T, Phys P, Dev D) {
+    constexpr int NCH = kMagPart / LPC;    // chunks of this lane: r = j + LPC * ch
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = gt / LPC;
+    const int j = gt & (LPC - 1);
     if (g >= ncell) return;
     const int c = cells[g];
     const int n = D.cn[c];
@@ -682,16 +682,48 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     const double dth = dt / 2.0, dtq = dt / 4.0, gS = P.gamma_S;
     const double rs = lang ? sqrt(2.0 * Tk * gS / dt) : 0.0;
     auto atom_at = [&](int q) { return lst[backward ? n - 1 - q : q]; };
+    auto load_ids = [&](int a, int (&id)[NCH][4]) {
+        const int4* p = reinterpret_cast<const int4*>(D.nbm + (size_t)a * kMaxMag);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int4 v = p[j + LPC * ch];
+            id[ch][0] = v.x; id[ch][1] = v.y; id[ch][2] = v.z; id[ch][3] = v.w;
+        }
+    };
+    auto load_js = [&](int a, double (&J)[NCH][4]) {
+        const double2* p = reinterpret_cast<const double2*>(D.nbJ + (size_t)a * kMaxMag);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const double2 u = p[2 * (j + LPC * ch)], v = p[2 * (j + LPC * ch) + 1];
+            J[ch][0] = u.x; J[ch][1] = u.y; J[ch][2] = v.x; J[ch][3] = v.y;
+        }
+    };
+    auto gather = [&](int a, int nm, const int (&id)[NCH][4], double4 (&sj)[NCH][4]) {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sj[ch][e] = D.s[4 * (j + LPC * ch) + e < nm ? id[ch][e] : a];
+    };
+    // sum over the lanes of one cell, in the order (P0 + P1) + (P2 + P3)
+    auto combine = [&](const double (&v)[NCH]) {
+        if constexpr (LPC == 1) return (v[0] + v[1]) + (v[2] + v[3]);
+        if constexpr (LPC == 2) {
+            const double u = v[0] + __shfl_xor(v[0], 1);
+            const double w = v[1] + __shfl_xor(v[1], 1);
+            return u + w;
+        }
+        const double u = v[0] + __shfl_xor(v[0], 1);
+        return u + __shfl_xor(u, 2);
+    };
 
     // pipeline registers: atom q (_c), atom q+1 (_n) and atom q+2's list entry
     int a_c = atom_at(0), nm_c = D.nbmn[a_c];
-    int id_c[kPipe], id_n[kPipe];
-    double J_c[kPipe];
-    double4 sj[kPipe];
-    load_ids(D, a_c, id_c);
-    load_js(D, a_c, J_c);
-#pragma unroll
-    for (int k = 0; k < kPipe; ++k) sj[k] = D.s[k < nm_c ? id_c[k] : a_c];
+    int id_c[NCH][4], id_n[NCH][4];
+    double J_c[NCH][4];
+    double4 sj[NCH][4];
+    load_ids(a_c, id_c);
+    load_js(a_c, J_c);
+    gather(a_c, nm_c, id_c, sj);
     const bool ext = P.extfield != 0;
     double4 own_c = D.s[a_c];
     double4 he_c = ext ? D.hext[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -702,7 +734,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     if (n > 1) {
         a_n = atom_at(1);
         nm_n = D.nbmn[a_n];
-        load_ids(D, a_n, id_n);
+        load_ids(a_n, id_n);
         rho_n = lang ? D.rho[a_n] : 0.0;
     }
     // the list entry two atoms ahead, so that no iteration waits for one
@@ -713,41 +745,52 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     double4 prev_old = prev_s;  // the previous atom's spin before its update
     for (int q = 0; q < n; ++q) {
         const int a = a_c;
-        // field of atom q (from Hext, then the gathered spins), then the
-        // previous atom's change of spin times its weight in the list
-        double hx = he_c.x, hy = he_c.y, hz = he_c.z, S = 0.0, Jp = 0.0;
+        // this lane's partial sums over the gathered spins (then its
+        // overflow entries, loaded now), and the previous atom's weight
+        double px[NCH], py[NCH], pz[NCH], ps[NCH];
+        double Jp = 0.0;
 #pragma unroll
-        for (int k = 0; k < kPipe; ++k) {
-            const double4 v = sj[k];
-            const double Jk = k < nm_c ? J_c[k] : 0.0;
-            hx += Jk * v.x; hy += Jk * v.y; hz += Jk * v.z;
-            S += Jk * v.w;
-            Jp = (k < nm_c && id_c[k] == prev_a) ? Jk : Jp;
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int k0 = 4 * (j + LPC * ch);
+            double x = 0.0, y = 0.0, z = 0.0, w = 0.0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double4 v = sj[ch][e];
+                const double Jk = k0 + e < nm_c ? J_c[ch][e] : 0.0;
+                x += Jk * v.x; y += Jk * v.y; z += Jk * v.z;
+                w += Jk * v.w;
+                Jp = (k0 + e < nm_c && id_c[ch][e] == prev_a) ? Jk : Jp;
+            }
+            for (int k = kPipe + k0; k < kPipe + k0 + 4 && k < nm_c; ++k) {
+                const double Jk = D.nbJ[(size_t)a * kMaxMag + k];
+                const double4 v = D.s[D.nbm[(size_t)a * kMaxMag + k]];
+                x += Jk * v.x; y += Jk * v.y; z += Jk * v.z;
+                w += Jk * v.w;
+            }
+            px[ch] = x; py[ch] = y; pz[ch] = z; ps[ch] = w;
         }
+        if constexpr (LPC >= 2) Jp += __shfl_xor(Jp, 1);
+        if constexpr (LPC >= 4) Jp += __shfl_xor(Jp, 2);
+        double hx = he_c.x + combine(px), hy = he_c.y + combine(py), hz = he_c.z + combine(pz);
+        double S = combine(ps);
         hx += Jp * (prev_s.x - prev_old.x);
         hy += Jp * (prev_s.y - prev_old.y);
         hz += Jp * (prev_s.z - prev_old.z);
         S += Jp * (prev_s.w - prev_old.w);
-        for (int k = kPipe; k < nm_c; ++k) {
-            double Jk = D.nbJ[(size_t)a * kMaxMag + k];
-            double4 v = D.s[D.nbm[(size_t)a * kMaxMag + k]];
-            hx += Jk * v.x; hy += Jk * v.y; hz += Jk * v.z;
-            S += Jk * v.w;
-        }
         const double3 H = make_double3(hx, hy, hz);
         prev_old = own_c;
         double3 s = d3(own_c);
-        const double3 z = d3(z_c);
+        const double3 zq = d3(z_c);
         const LanRows lan = lan_c;
         // pipeline: issue atom q+1's neighbour spins, J values, own spin,
         // noise and Landau rows, and atom q+2's ids and density
         if (q + 1 < n) {
 #pragma unroll
-            for (int k = 0; k < kPipe; ++k) {
-                id_c[k] = id_n[k];
-                sj[k] = D.s[k < nm_n ? id_n[k] : a_n];
-            }
-            load_js(D, a_n, J_c);
+            for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) id_c[ch][e] = id_n[ch][e];
+            gather(a_n, nm_n, id_n, sj);
+            load_js(a_n, J_c);
             own_c = D.s[a_n];
             if (ext) he_c = D.hext[a_n];
             if (lang) {
@@ -759,7 +802,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             if (q + 2 < n) {
                 a_n = a_nn;
                 nm_n = D.nbmn[a_n];
-                load_ids(D, a_n, id_n);
+                load_ids(a_n, id_n);
                 if (lang) rho_n = D.rho[a_n];
                 if (q + 3 < n) a_nn = atom_at(q + 3);
             }
@@ -779,21 +822,30 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             double3 st = add3(s, mul3(gS * dtq, add3(H, L)));
             L = landau_field(st, A, B, C, Dd, S);
             s = add3(s, mul3(gS * dth, add3(H, L)));
-            s = add3(s, mul3(dt, mul3(rs, z)));
+            s = add3(s, mul3(dt, mul3(rs, zq)));
             L = landau_field(s, A, B, C, Dd, S);
             st = add3(s, mul3(gS * dtq, add3(H, L)));
             L = landau_field(st, A, B, C, Dd, S);
             s = add3(s, mul3(gS * dth, add3(H, L)));
             s = apply_rot(R, s);
-            D.sjs[a] = S;
         }
         prev_s = make_double4(s.x, s.y, s.z, spin_len(s));
         prev_a = a;
-        D.s[a] = prev_s;
-        D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
-        D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
+        if (j == 0) {
+            D.s[a] = prev_s;
+            // the fields of the forward sweep are overwritten by the
+            // backward sweep before anything reads them
+            if (store_fields) {
+                if (lang) D.sjs[a] = S;
+                D.hh[a] = make_double4(H.x, H.y, H.z, 0.0);
+                D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
+            }
+        }
     }
 }
+template __global__ void k_spin_phase<1>(const int*, int, int, int, double, Tables, Phys, Dev);
+template __global__ void k_spin_phase<2>(const int*, int, int, int, double, Tables, Phys, Dev);
+template __global__ void k_spin_phase<4>(const int*, int, int, int, double, Tables, Phys, Dev);
 
 // ---------------------------------------------------------------------------
 // momenta (core_dp_CPU.cpp:51-312). Every part of the Langevin update is local
@@ -1452,13 +1504,6 @@ __global__ __launch_bounds__(256) void k_renorm(Dev D, const double* err, double
         acc[0] = te;
     }
     block_partials<1>(acc, partials);
-}
-
-// Ee = E(Te) over the cells [c0, c0 + n) (map_cells.cpp:150-157)
-__global__ __launch_bounds__(256) void k_cells_ee(Dev D, int c0, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    D.Ee[c0 + k] = te_to_ee(D.Te[c0 + k]);
 }
 
 // .w = |s| over the atom slots [a0, a0 + n), after the host uploaded spins

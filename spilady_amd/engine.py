@@ -237,7 +237,8 @@ class Engine(abi.EngineLike):
         self._rc(self.L.sld_time_kernel(self.h, (name or "").encode()))
 
     def kernel_timing(self):
-        """(launches, total device ms) of the timed kernel over the last step call."""
+        """(launches, total device ms) of the timed kernel over the last step or
+        core_dTe call."""
         n = C.c_int(0)
         ms = C.c_double(0.0)
         self._rc(self.L.sld_kernel_timing(self.h, C.byref(n), C.byref(ms)))

@@ -15,10 +15,13 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("r", "p", "f", "s", "rho", "pe", "ke", "me", "me0", "stress", "heff_h", "heff_l")
 # sweep_tail 0 launches every diffusion level on its own; the default (2048)
-# and 8192 run the trailing levels of each sweep in one workgroup
+# and 8192 run the trailing levels of each sweep in one workgroup; spin_lanes
+# 1 and 2 spread a cell's spin walk over fewer lanes than the default 4
 OTHER_SHAPES = [
-    dict(spin_block=32, sweep_block=256, force_block=128, nbuild_cells=1, temperature_block=64, sweep_tail=0),
-    dict(spin_block=128, sweep_block=128, force_block=64, nbuild_cells=4, temperature_block=128, sweep_tail=8192),
+    dict(spin_block=32, sweep_block=256, force_block=128, nbuild_cells=1, temperature_block=64, sweep_tail=0,
+         spin_lanes=1),
+    dict(spin_block=128, sweep_block=128, force_block=64, nbuild_cells=4, temperature_block=128, sweep_tail=8192,
+         spin_lanes=2),
 ]
 
 
