@@ -590,7 +590,12 @@ __global__ void k_obs_set_te(const double* te_sum, int C, sld_obs* out) {
 
 namespace {
 
-int do_check(sld_ctx* c, int step, sld_obs* d_out) {
+// check() of the reference (check.cpp:31-47). With renormalizeEnergy the
+// reference's energy check also rewrites every cell's Ee and Te, at step -1
+// too (Te = T(E(Te)), check_energy_CPU.cpp:100-111); feedback = false computes
+// the observables only (sld_create's initial record, which the reference's
+// initialize() does not make).
+int do_check(sld_ctx* c, int step, sld_obs* d_out, bool feedback = true) {
     int na = nblk(c->D.N, 256);
     int scan = c->phys.eltemp ? 0 : 1;
     if (scan) TRY(ensure_nb(c));
@@ -603,7 +608,7 @@ int do_check(sld_ctx* c, int step, sld_obs* d_out) {
     }
     LAUNCH(c, k_obs_final, nblk(c->R, 64), 64, (const double*)c->d_scal, (const Rep*)c->D.rep, c->R,
            c->phys.eltemp, c->prm.renormalize_energy, step, c->total_time, c->d_einit, d_out);
-    if (c->phys.eltemp && c->prm.renormalize_energy) {
+    if (feedback && c->phys.eltemp && c->prm.renormalize_energy) {
         int nb = nblk(c->D.C, 256);
         LAUNCH(c, k_renorm, nb, 256, c->D, (const double*)(c->d_einit + 1), c->d_part);
         TRY(reduce(c, nb, 1, c->d_scal + 44));
@@ -1250,7 +1255,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     c->pass_dev = 0;
     c->cl_first = c->D.cl;
     c->last_t_stress = c->last_t_pressure = c->total_time;  // initial.cpp:93-98
-    TRY(do_check(c, -1, c->d_obs));
+    TRY(do_check(c, -1, c->d_obs, false));
     HIPCHK(c, hipMemcpyAsync(&c->last_obs, c->d_obs, sizeof(sld_obs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return read_flags(c);
