@@ -321,6 +321,18 @@ int sld_model_eval(sld_ctx* ctx, double* energy, double* force, double* field);
  * NULL upstream gradients count as zero. */
 int sld_model_vjp(sld_ctx* ctx, const double* g_energy, const double* g_force, const double* g_field,
                   double* grad);
+/* sld_model_eval for callers that cannot allocate device memory (code under
+ * stream capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference. force and field are device buffers (or
+ * NULL), written directly; energy may be host or device. It allocates
+ * nothing: its scratch (density and F'(rho) per atom slot, energy partials)
+ * is a fixed array in the library's device image, 4.2 MB on each GPU it runs
+ * on, held until the process ends. Contexts of more than 262,144 atom slots
+ * (padding included) or 64 replicas are refused with SLD_ERR_CAPACITY. All
+ * calls on one GPU share that scratch: calls that overlap on one GPU (other
+ * streams or host threads) race. Results are bitwise those of
+ * sld_model_eval. */
+int sld_model_eval_noalloc(sld_ctx* ctx, double* energy, double* force, double* field);
 
 #ifdef __cplusplus
 }

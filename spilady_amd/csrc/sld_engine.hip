@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1924,3 +1925,83 @@ int sld_model_natoms(const sld_ctx* c) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// sld_model_eval_noalloc: sld_model_eval for callers that cannot allocate
+// device memory. Its scratch (densities and F'(rho) per atom slot, the block
+// partials of the energy and the per-replica sums) lives in the file-scope
+// array below, of a fixed size, which every GPU the library runs on holds
+// until the process ends. Host code reaches a device's copy only through the
+// address hipGetSymbolAddress returns for that device.
+namespace {
+constexpr int kNoAllocAtoms = 1 << 18;                   // atom slots, padding included
+constexpr int kNoAllocBlocks = kNoAllocAtoms / 256 + 8;  // 256-thread blocks of the energy pass
+constexpr int kNoAllocReps = 64;                         // replicas
+constexpr size_t kNoAllocDoubles =
+    2 * (size_t)kNoAllocAtoms + (size_t)kNoAllocBlocks + (size_t)kScal * kNoAllocReps;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+// the current device's copy of `scratch`, looked up once per device
+int noalloc_scratch(sld_ctx* c, double** out) {
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_err(c, SLD_ERR_ARG, "device %d outside the scratch table", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(c, SLD_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                           hipGetErrorString(e));
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return SLD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sld_model_eval_noalloc(sld_ctx* c, double* energy, double* force, double* field) {
+    if (!c) return SLD_ERR_ARG;
+    auto& m = c->model;
+    if (!m.ready) return set_err(c, SLD_ERR_STATE, "model parameters not set (sld_model_set_params)");
+    const int N = c->D.N, nb = nblk(N, 256);
+    if (N > kNoAllocAtoms || nb > kNoAllocBlocks || c->R > kNoAllocReps)
+        return set_err(c, SLD_ERR_CAPACITY,
+                       "sld_model_eval_noalloc: %d atom slots in %d replicas exceed its fixed scratch "
+                       "(%d atom slots, %d replicas); use sld_model_eval",
+                       N, c->R, kNoAllocAtoms, kNoAllocReps);
+    double* base = nullptr;
+    TRY(noalloc_scratch(c, &base));
+    double* rho = base;
+    double* fp = base + kNoAllocAtoms;
+    double* part = base + 2 * (size_t)kNoAllocAtoms;
+    double* esum = part + kNoAllocBlocks;
+    c->ev_used = 0;
+    // every scratch element read below is written earlier in this call: the
+    // density pass writes rho and fp of every real atom slot (the only ones
+    // the energy pass reads), the energy pass one partial per block, the
+    // segmented sum one value per replica
+    TRY(ensure_nb(c));
+    TRY(read_flags(c));
+    LAUNCH(c, k_model_rho, xcd_grid(nb), 256, m.tab, c->phys, c->D, rho, fp);
+    double* F = force ? force : m.d_F;
+    double* H = field ? field : m.d_H;
+    LAUNCH(c, k_model_forward, xcd_grid(nb), 256, m.tab, c->phys, c->D, (const double*)rho, (const double*)fp,
+           (const int*)m.d_coff, F, H, part);
+    LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)part, (const int*)c->d_atom_seg, 1, esum, 0);
+    if (energy)
+        HIPCHK(c, hipMemcpy2DAsync(energy, sizeof(double), esum, kScal * sizeof(double), sizeof(double), c->R,
+                                   hipMemcpyDefault, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+}
+
+}  // extern "C"

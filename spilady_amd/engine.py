@@ -130,6 +130,7 @@ def load(path: Optional[str] = None):
     # buffers are raw addresses: host arrays or device tensors
     L.sld_model_eval.argtypes = [V, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sld_model_vjp.argtypes = [V, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sld_model_eval_noalloc.argtypes = [V, C.c_void_p, C.c_void_p, C.c_void_p]
     if path is None:
         _lib = L
     return L
