@@ -115,10 +115,18 @@ def nve_drift(nsteps: int, every: int = 100):
                  [o["ave_pe"] + o["ave_ke"] + o["ave_me"] for o in obs])
     t = np.array([0.0] + [o["total_time"] for o in obs]) * 1e12     # ps
     slope = float(np.polyfit(t, e, 1)[0])
+    # after the first picosecond, once the initial swing of the splitting
+    # error has settled
+    late = t >= 1.0 - 1e-9
+    fit = np.polyfit(t[late], e[late], 1) if late.sum() >= 3 else None
     return {"metric": "NVE energy drift, 250k-atom bcc Fe SLD (FP64)", "steps": nsteps,
             "dt_fs": p.step * 1e15, "e0_ev_per_atom": float(e[0]), "e_end_ev_per_atom": float(e[-1]),
             "max_abs_dev_ev_per_atom": float(np.max(np.abs(e - e[0]))),
-            "drift_ev_per_atom_per_ps": slope, "ke_ev_per_atom": float(obs[-1]["ave_ke"]),
+            "drift_ev_per_atom_per_ps": slope,
+            "drift_after_1ps_ev_per_atom_per_ps": None if fit is None else float(fit[0]),
+            "std_about_fit_after_1ps_ev_per_atom":
+                None if fit is None else float(np.std(e[late] - np.polyval(fit, t[late]))),
+            "ke_ev_per_atom": float(obs[-1]["ave_ke"]),
             "ms_per_step": wall / nsteps * 1e3, "atom_steps_per_s": at.natom * nsteps / wall,
             "series_every": every, "series": e.tolist()}
 
