@@ -326,10 +326,17 @@ int sync_pass(sld_ctx* c) {
     return SLD_OK;
 }
 
+// the Landau coefficients follow the density
+int refresh_landau(sld_ctx* c) {
+    LAUNCH(c, k_landau, nblk(c->D.N, 256), 256, c->tab, c->D);
+    return SLD_OK;
+}
+
 int nbuild(sld_ctx* c, int write_rho) {
     const int w = c->shapes.nbuild_cells;
     LAUNCH(c, k_nbuild, xcd_grid(nblk(c->D.C, w)), 64 * w, c->tab, c->phys, c->D, write_rho);
     c->nb_dirty = false;
+    if (write_rho) TRY(refresh_landau(c));
     return SLD_OK;
 }
 
@@ -390,7 +397,8 @@ int do_dp(sld_ctx* c, double dt) {
             LAUNCH(c, k_qn_refresh, nblk(c->D.N, 256), 256, c->phys, c->D, c->qcalls / (uint64_t)c->phys.qm);
         ++c->qcalls;
     }
-    LAUNCH(c, k_dp_cells, nb, 256, dt, c->phys, c->D, (int)(c->pass - c->pass_dev), c->d_part);
+    // eight cells (8 lanes each) per 64-thread block
+    LAUNCH(c, k_dp_cells, nb, 64, dt, c->phys, c->D, (int)(c->pass - c->pass_dev), c->d_part);
     if (c->phys.lattlang) c->pass++;
     TRY(reduce_seg(c, c->d_cell8_seg, 3, 0));
     int na = nblk(c->D.N, 256);
@@ -409,7 +417,7 @@ int do_dr(sld_ctx* c, double dt) {
 int do_temperature(sld_ctx* c) {
     TRY(ensure_nb(c));
     const int bs = c->shapes.temperature_block;
-    LAUNCH(c, k_temperature, xcd_grid(nblk(c->D.C, bs / 32)), bs, c->tab, c->phys, c->D);
+    LAUNCH(c, k_temperature, xcd_grid(nblk(c->D.C, bs / kSeg)), bs, c->tab, c->phys, c->D);
     return SLD_OK;
 }
 
@@ -1116,6 +1124,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     TRY(dalloc(c, &D.r, N)); TRY(dalloc(c, &D.p, N)); TRY(dalloc(c, &D.f, N));
     TRY(dalloc(c, &D.s, N)); TRY(dalloc(c, &D.hh, N)); TRY(dalloc(c, &D.hl, N));
     TRY(dalloc(c, &D.rho, N)); TRY(dalloc(c, &D.pe, N)); TRY(dalloc(c, &D.ke, N));
+    TRY(dalloc(c, &D.lan, N));
     TRY(dalloc(c, &D.me, N)); TRY(dalloc(c, &D.me0, N)); TRY(dalloc(c, &D.vir, N));
     TRY(dalloc(c, &D.zn, N));
     if (P0.extfield) TRY(dalloc(c, &D.hext, N));  // zero: the reference's default field
@@ -1247,8 +1256,9 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
         TRY(up1(c, D.Te, te.data(), 0, C));
         TRY(up1(c, D.Ee, ee.data(), 0, C));
     }
-    // initial.cpp:78-87
+    // initial.cpp:78-87 (with the densities of the input when they are not computed)
     TRY(nbuild(c, compute_forces ? 1 : 0));
+    if (!compute_forces) TRY(refresh_landau(c));
     if (compute_forces) TRY(do_force(c));
     if (P0.eltemp) TRY(do_temperature(c));
     c->total_time = P0.start_time;
@@ -1490,6 +1500,7 @@ int sld_upload(sld_ctx* c, const sld_atoms* in) {
     TRY(up4(c, D.s, in->s, r, true));
     TRY(up4(c, D.f, in->f, r, false));
     TRY(up1(c, D.rho, in->rho, R.atom0, R.natom));
+    TRY(refresh_landau(c));
     if (in->r) c->nb_dirty = true;
     return SLD_OK;
 }

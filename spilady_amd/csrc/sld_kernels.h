@@ -42,6 +42,7 @@ struct Dev {
     double *qring, *qval;
     const double* qH;
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
+    double4* lan;              // Landau A, B, C, D at each atom's density [N] (kept current with rho)
     double* stress;            // [6][N]: 11 22 33 12 23 31
     int* cell;                 // link cell of each atom (new_cell_index)
     int* cell_prev;            // link cell before the last links()
@@ -77,6 +78,9 @@ struct Phys {
 
 // per-replica block of reduced scalars
 constexpr int kScal = 64;
+// lanes per link cell in the momentum and temperature passes (a lane takes
+// the cell's list entries l, l + kSeg, ...)
+constexpr int kSeg = 8;
 
 // ---- kernels ----
 __global__ void k_bin(Dev D);
@@ -85,6 +89,7 @@ __global__ void k_nbuild(Tables T, Phys P, Dev D, int write_rho);
 __global__ void k_fp(Tables T, Dev D);
 __global__ void k_force(Tables T, Phys P, Dev D);
 __global__ void k_spin_noise(Dev D, int pass_off);
+__global__ void k_landau(Tables T, Dev D);
 __global__ void k_step_max(Dev D, double* partials);
 __global__ void k_scale_r(Dev D, double factor);
 __global__ void k_rebox_r(Dev D, Box old_box, Box new_box);

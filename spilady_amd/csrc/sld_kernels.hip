@@ -453,10 +453,8 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     double ratio;
     interp_index(D.rho[i], T.in_over_rhomax, n0, ratio);
     double F = lerp_tab(T.rhotab, kRhoStride, H_F, n0, ratio);
-    double A = lerp_tab(T.rhotab, kRhoStride, H_A, n0, ratio);
-    double B = lerp_tab(T.rhotab, kRhoStride, H_B, n0, ratio);
-    double C = lerp_tab(T.rhotab, kRhoStride, H_C, n0, ratio);
-    double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
+    const double4 lan = D.lan[i];
+    const double A = lan.x, B = lan.y, C = lan.z, Dd = lan.w;
     double s4 = si_sq * si_sq;
     pe += F;
     if (P.extfield) me -= dot3(si, d3(D.hext[i]));  // calculate_force_energy_CPU.cpp:199-201
@@ -665,6 +663,21 @@ __device__ __forceinline__ LanRows landau_rows(const Tables& T, double rho) {
     return L;
 }
 
+// The Landau coefficients A..D of every atom at its current density
+// (calculate_spin_CPU.cpp:118-121), interpolated as the reference does. The
+// density changes once a step (calculate_rho); the four spin sweeps, the
+// force pass and the temperature and energy observables in between read the
+// coefficients from here, 32 contiguous bytes per atom, instead of two
+// 128-byte table rows each.
+__global__ __launch_bounds__(256) void k_landau(Tables T, Dev D) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.N || D.arep[i] < 0) return;
+    const LanRows L = landau_rows(T, D.rho[i]);
+    const double w = L.ratio, v = 1.0 - L.ratio;
+    D.lan[i] = make_double4(L.lo.x * w + L.hi.x * v, L.lo.y * w + L.hi.y * v, L.lo.z * w + L.hi.z * v,
+                            L.lo.w * w + L.hi.w * v);
+}
+
 template <int LPC>
 __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell, int backward, int store_fields,
                                                     double dt, Tables T, Phys P, Dev D) {
@@ -728,14 +741,12 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
     double4 own_c = D.s[a_c];
     double4 he_c = ext ? D.hext[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
     double4 z_c = lang ? D.zn[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
-    LanRows lan_c = lang ? landau_rows(T, D.rho[a_c]) : LanRows{};
+    double4 lan_c = lang ? D.lan[a_c] : make_double4(0.0, 0.0, 0.0, 0.0);
     int a_n = a_c, nm_n = nm_c;
-    double rho_n = 0.0;
     if (n > 1) {
         a_n = atom_at(1);
         nm_n = D.nbmn[a_n];
         load_ids(a_n, id_n);
-        rho_n = lang ? D.rho[a_n] : 0.0;
     }
     // the list entry two atoms ahead, so that no iteration waits for one
     // before it can request the next atom's ids
@@ -781,7 +792,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
         prev_old = own_c;
         double3 s = d3(own_c);
         const double3 zq = d3(z_c);
-        const LanRows lan = lan_c;
+        const double4 lan = lan_c;
         // pipeline: issue atom q+1's neighbour spins, J values, own spin,
         // noise and Landau rows, and atom q+2's ids and density
         if (q + 1 < n) {
@@ -795,7 +806,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
             if (ext) he_c = D.hext[a_n];
             if (lang) {
                 z_c = D.zn[a_n];
-                lan_c = landau_rows(T, rho_n);
+                lan_c = D.lan[a_n];
             }
             a_c = a_n;
             nm_c = nm_n;
@@ -803,7 +814,6 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
                 a_n = a_nn;
                 nm_n = D.nbmn[a_n];
                 load_ids(a_n, id_n);
-                if (lang) rho_n = D.rho[a_n];
                 if (q + 3 < n) a_nn = atom_at(q + 3);
             }
         }
@@ -811,11 +821,7 @@ __global__ __launch_bounds__(256) void k_spin_phase(const int* cells, int ncell,
         if (!lang) {
             s = apply_rot(make_rot(H, dt), s);
         } else {
-            const double w = lan.ratio, v = 1.0 - lan.ratio;
-            const double A = lan.lo.x * w + lan.hi.x * v;
-            const double B = lan.lo.y * w + lan.hi.y * v;
-            const double C = lan.lo.z * w + lan.hi.z * v;
-            const double Dd = lan.lo.w * w + lan.hi.w * v;
+            const double A = lan.x, B = lan.y, C = lan.z, Dd = lan.w;
             const Rot R = make_rot(H, dth);
             s = apply_rot(R, s);
             L = landau_field(s, A, B, C, Dd, S);
@@ -849,41 +855,59 @@ template __global__ void k_spin_phase<4>(const int*, int, int, int, double, Tabl
 
 // ---------------------------------------------------------------------------
 // momenta (core_dp_CPU.cpp:51-312). Every part of the Langevin update is local
-// to one link cell, so a thread runs C1, B, A, B, C2 for its cell in order.
+// to one link cell, so one group of kSeg lanes runs C1, B, A, B, C2 for its
+// cell in order. Lane l of a group holds the cell's list entries l, l + kSeg,
+// ... (kSegSlots slots; bcc Fe has about six atoms per cell, so most lanes
+// hold one): eight cells per wavefront.
 
-__device__ __forceinline__ double seg_sum32(double v) {
+constexpr int kSegSlots = kCellCap / kSeg;
+static_assert(kCellCap % kSeg == 0, "a cell's list fills whole slots");
+
+// sum over a cell: each lane's slots in order, then a butterfly over the group
+__device__ __forceinline__ double seg_sum(const double (&v)[kSegSlots]) {
+    double t = v[0];
 #pragma unroll
-    for (int m = 16; m > 0; m >>= 1) v += __shfl_xor(v, m, 32);
-    return v;
+    for (int s = 1; s < kSegSlots; ++s) t += v[s];
+#pragma unroll
+    for (int m = kSeg / 2; m > 0; m >>= 1) t += __shfl_xor(t, m, kSeg);
+    return t;
 }
 
-__device__ __forceinline__ double3 seg_sum32(double3 v) {
-    return make_double3(seg_sum32(v.x), seg_sum32(v.y), seg_sum32(v.z));
+__device__ __forceinline__ double3 seg_sum(const double3 (&v)[kSegSlots]) {
+    double x[kSegSlots], y[kSegSlots], z[kSegSlots];
+#pragma unroll
+    for (int s = 0; s < kSegSlots; ++s) {
+        x[s] = v[s].x; y[s] = v[s].y; z[s] = v[s].z;
+    }
+    return make_double3(seg_sum(x), seg_sum(y), seg_sum(z));
 }
 
 // Local collective motion step: walking the list (head to tail, or tail to
 // head), each atom gets p += g dt/n (sum_p - p) with sum_p over the cell's
-// current momenta (core_dp_CPU.cpp:211-264). Lane l holds list entry l.
-// The cell sum is carried along: after atom k changes by d, the sum changes
-// by d (the reference re-adds the list each time; same value up to rounding).
-__device__ __forceinline__ double3 dp_colmot(double3 p, int l, int n, double factor, bool backward) {
-    double3 sp = seg_sum32(p);
-    const int seg = threadIdx.x & 32;
+// current momenta (core_dp_CPU.cpp:211-264). The cell sum is carried along:
+// after atom k changes by d, the sum changes by d (the reference re-adds the
+// list each time; same value up to rounding).
+__device__ __forceinline__ void dp_colmot(double3 (&p)[kSegSlots], int l, int n, double factor, bool backward) {
+    double3 sp = seg_sum(p);
+    const int base = (threadIdx.x & 63) & ~(kSeg - 1);
     for (int q = 0; q < n; ++q) {
         const int k = backward ? n - 1 - q : q;
+        const int kl = k % kSeg, ks = k / kSeg;
         double3 d = make_double3(0.0, 0.0, 0.0);
-        if (l == k) {
-            double3 pn = add3(p, mul3(factor, sub3(sp, p)));
-            d = sub3(pn, p);
-            p = pn;
+#pragma unroll
+        for (int s = 0; s < kSegSlots; ++s) {
+            if (l == kl && s == ks) {
+                const double3 pn = add3(p[s], mul3(factor, sub3(sp, p[s])));
+                d = sub3(pn, p[s]);
+                p[s] = pn;
+            }
         }
-        const int src = seg + k;
+        const int src = base + kl;
         d.x = __shfl(d.x, src, 64);
         d.y = __shfl(d.y, src, 64);
         d.z = __shfl(d.z, src, 64);
         sp = add3(sp, d);
     }
-    return p;
 }
 
 // the force of the momentum update: with extforce the reference adds fext to
@@ -954,54 +978,81 @@ __global__ __launch_bounds__(256) void k_qn_refresh(Phys P, Dev D, uint64_t r) {
     q[2] = n2;
 }
 
-// One half-wave (32 lanes) per link cell, one atom per lane.
-__global__ __launch_bounds__(256) void k_dp_cells(double dt, Phys P, Dev D, int pass_off,
-                                                  double* partials) {
-    const int l = threadIdx.x & 31;
-    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+// One group of kSeg lanes per link cell, eight cells per 64-thread block (a
+// replica's cells start on an 8-cell boundary, so each block's partial sums
+// belong to one replica).
+__global__ __launch_bounds__(64) void k_dp_cells(double dt, Phys P, Dev D, int pass_off, double* partials) {
+    const int l = threadIdx.x & (kSeg - 1);
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) / kSeg;
     const int rr = c < D.C ? D.crep[c] : -1;
     const bool valid = rr >= 0;
     const int n = valid ? D.cn[c] : 0;
-    const bool has = l < n;
-    const int a = has ? D.cl[c * kCellCap + l] : 0;
-    double3 p = has ? d3(D.p[a]) : make_double3(0.0, 0.0, 0.0);
+    bool has[kSegSlots];
+    int a[kSegSlots];
+    double3 p[kSegSlots];
+#pragma unroll
+    for (int s = 0; s < kSegSlots; ++s) {
+        has[s] = l + kSeg * s < n;
+        a[s] = has[s] ? D.cl[c * kCellCap + l + kSeg * s] : 0;
+        p[s] = has[s] ? d3(D.p[a[s]]) : make_double3(0.0, 0.0, 0.0);
+    }
     if (valid) {
         const double gm = P.gamma_L / P.atmass;
         if (P.lattlang) {
             const double h = dt / 2.0;
-            if (P.localcolmot && n > 0) p = dp_colmot(p, l, n, gm * h / n, false);
+            if (P.localcolmot && n > 0) dp_colmot(p, l, n, gm * h / n, false);
             const double eb = P.localcolmot ? exp(-gm * h * (1.0 - 1.0 / n)) : exp(-gm * h);
-            p = mul3(eb, p);
+#pragma unroll
+            for (int s = 0; s < kSegSlots; ++s) p[s] = mul3(eb, p[s]);
             const Rep& R = D.rep[rr];
             const double Tk = P.eltemp ? D.Te[c] : R.temperature;
-            double3 ff = make_double3(0.0, 0.0, 0.0);
-            if (has) {
-                if (P.quantumnoise) {
-                    // core_dp_CPU.cpp:137-146: coloured noise scaled by sqrt(2 gamma / h), h = M dt
-                    const double hq = dt * P.qm;
-                    const double len = sqrt(2.0 * P.gamma_L / hq);
-                    const double* q = D.qval + 3 * (size_t)a;
-                    ff = mul3(len, make_double3(q[0], q[1], q[2]));
-                } else {
-                    const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
-                    const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
-                    ff = mul3(len, normal3(R.seed, R.replica, a - R.atom0, pass));
+            double3 ff[kSegSlots];
+#pragma unroll
+            for (int s = 0; s < kSegSlots; ++s) {
+                ff[s] = make_double3(0.0, 0.0, 0.0);
+                if (has[s]) {
+                    if (P.quantumnoise) {
+                        // core_dp_CPU.cpp:137-146: coloured noise scaled by sqrt(2 gamma / h), h = M dt
+                        const double hq = dt * P.qm;
+                        const double len = sqrt(2.0 * P.gamma_L / hq);
+                        const double* q = D.qval + 3 * (size_t)a[s];
+                        ff[s] = mul3(len, make_double3(q[0], q[1], q[2]));
+                    } else {
+                        const double len = sqrt(2.0 * Tk * P.gamma_L / dt);
+                        const uint64_t pass = D.pass_base[0] + (uint64_t)pass_off;
+                        ff[s] = mul3(len, normal3(R.seed, R.replica, a[s] - R.atom0, pass));
+                    }
+                    p[s] = add3(p[s], mul3(dt, add3(dp_force(P, D, a[s]), ff[s])));
                 }
-                p = add3(p, mul3(dt, add3(dp_force(P, D, a), ff)));
             }
             if (P.localcolmot && n > 0) {
-                double3 fs = seg_sum32(ff);
-                double3 ave = make_double3(fs.x / n, fs.y / n, fs.z / n);
-                if (has) p = sub3(p, mul3(dt, ave));
+                const double3 fs = seg_sum(ff);
+                const double3 ave = make_double3(fs.x / n, fs.y / n, fs.z / n);
+#pragma unroll
+                for (int s = 0; s < kSegSlots; ++s)
+                    if (has[s]) p[s] = sub3(p[s], mul3(dt, ave));
             }
-            p = mul3(eb, p);
-            if (P.localcolmot && n > 0) p = dp_colmot(p, l, n, gm * h / n, true);
-        } else if (has) {
-            p = add3(p, mul3(dt, dp_force(P, D, a)));
+#pragma unroll
+            for (int s = 0; s < kSegSlots; ++s) p[s] = mul3(eb, p[s]);
+            if (P.localcolmot && n > 0) dp_colmot(p, l, n, gm * h / n, true);
+        } else {
+#pragma unroll
+            for (int s = 0; s < kSegSlots; ++s)
+                if (has[s]) p[s] = add3(p[s], mul3(dt, dp_force(P, D, a[s])));
         }
-        if (has) D.p[a] = make_double4(p.x, p.y, p.z, 0.0);
+#pragma unroll
+        for (int s = 0; s < kSegSlots; ++s)
+            if (has[s]) D.p[a[s]] = make_double4(p[s].x, p[s].y, p[s].z, 0.0);
     }
-    double acc[3] = {has ? p.x : 0.0, has ? p.y : 0.0, has ? p.z : 0.0};
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < kSegSlots; ++s) {
+        if (has[s]) {
+            acc[0] += p[s].x;
+            acc[1] += p[s].y;
+            acc[2] += p[s].z;
+        }
+    }
     block_partials<3>(acc, partials);
 }
 
@@ -1054,42 +1105,43 @@ __global__ __launch_bounds__(256) void k_dr(double dt, Phys P, Dev D) {
 // ---------------------------------------------------------------------------
 // electron subsystem
 
-// calculate_temperature_CPU.cpp:35-148: one half-wave per link cell, one atom
-// per lane; the cell sums are lane reductions.
+// calculate_temperature_CPU.cpp:35-148: one group of kSeg lanes per link
+// cell, lane l taking the cell's list entries l, l + kSeg, ...; the cell sums
+// are each lane's entries in order, then a butterfly over the group.
+__device__ __forceinline__ double seg_reduce(double t) {
+#pragma unroll
+    for (int m = kSeg / 2; m > 0; m >>= 1) t += __shfl_xor(t, m, kSeg);
+    return t;
+}
+
 __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
-    const int l = threadIdx.x & 31;
-    const int lb = xcd_block((D.C * 32 + blockDim.x - 1) / blockDim.x);
+    const int l = threadIdx.x & (kSeg - 1);
+    const int lb = xcd_block((D.C * kSeg + blockDim.x - 1) / blockDim.x);
     if (lb < 0) return;
-    const int c = (lb * blockDim.x + threadIdx.x) >> 5;
+    const int c = (lb * blockDim.x + threadIdx.x) / kSeg;
     if (c >= D.C || D.crep[c] < 0) return;
     const int n = D.cn[c];
-    const bool has = l < n;
-    double ke = 0.0, rup = 0.0, rdn = 0.0, lup = 0.0, ldn = 0.0;
-    double3 pa = make_double3(0.0, 0.0, 0.0);
-    if (has) {
-        const int a = D.cl[c * kCellCap + l];
-        ke = D.ke[a];
-        pa = d3(D.p[a]);
+    double ke = 0.0, rup = 0.0, rdn = 0.0, lup = 0.0, ldn = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    for (int k = l; k < n; k += kSeg) {
+        const int a = D.cl[c * kCellCap + k];
+        ke += D.ke[a];
+        const double3 pa = d3(D.p[a]);
+        px += pa.x; py += pa.y; pz += pa.z;
         double3 H;
         double S;
         exchange_field(a, D, P, H, S);
         double3 s = d3(D.s[a]);
-        rup = sq3(cross3(s, H));
-        rdn = dot3(s, H);
-        int n0;
-        double ratio;
-        interp_index(D.rho[a], T.in_over_rhomax, n0, ratio);
-        double A = lerp_tab(T.rhotab, kRhoStride, H_A, n0, ratio);
-        double B = lerp_tab(T.rhotab, kRhoStride, H_B, n0, ratio);
-        double C = lerp_tab(T.rhotab, kRhoStride, H_C, n0, ratio);
-        double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
+        rup += sq3(cross3(s, H));
+        rdn += dot3(s, H);
+        const double4 lan = D.lan[a];
+        const double A = lan.x, B = lan.y, C = lan.z, Dd = lan.w;
         double s2 = sq3(s), s4 = s2 * s2;
         double s0 = sqrt(s2);
-        double k = -(2.0 * A + 4.0 * B * s2 + 6.0 * C * s4 + 8.0 * Dd * s4 * s2);
+        double kk = -(2.0 * A + 4.0 * B * s2 + 6.0 * C * s4 + 8.0 * Dd * s4 * s2);
         double hc = -S / s0;
-        double3 L = make_double3((k + hc) * s.x, (k + hc) * s.y, (k + hc) * s.z);
-        lup = sq3(add3(H, L));
-        ldn = 6.0 * A + 20.0 * B * s2 + 42.0 * C * s4 + 72.0 * Dd * s4 * s2 + 2.0 * S / s0;
+        double3 L = make_double3((kk + hc) * s.x, (kk + hc) * s.y, (kk + hc) * s.z);
+        lup += sq3(add3(H, L));
+        ldn += 6.0 * A + 20.0 * B * s2 + 42.0 * C * s4 + 72.0 * Dd * s4 * s2 + 2.0 * S / s0;
         // s0 stays local: the reference stores it (calculate_temperature_CPU.cpp:100),
         // but here .w is read by the other cells' lanes of this launch through
         // exchange_field, so writing it would race with those reads. The spin
@@ -1098,12 +1150,12 @@ __global__ __launch_bounds__(256) void k_temperature(Tables T, Phys P, Dev D) {
         D.hl[a] = make_double4(L.x, L.y, L.z, 0.0);
         D.sjs[a] = S;
     }
-    double sum_ke = seg_sum32(ke);
-    double3 ap = seg_sum32(pa);
-    rup = seg_sum32(rup);
-    rdn = seg_sum32(rdn);
-    lup = seg_sum32(lup);
-    ldn = seg_sum32(ldn);
+    double sum_ke = seg_reduce(ke);
+    double3 ap = make_double3(seg_reduce(px), seg_reduce(py), seg_reduce(pz));
+    rup = seg_reduce(rup);
+    rdn = seg_reduce(rdn);
+    lup = seg_reduce(lup);
+    ldn = seg_reduce(ldn);
     if (l != 0) return;
     const Rep& R = D.rep[D.crep[c]];
     const double apc = (double)R.natom / (double)R.ncell;
@@ -1455,13 +1507,8 @@ __global__ __launch_bounds__(256) void k_obs_atoms(Tables T, Phys P, Dev D, doub
             exchange_field(a, D, P, H, S);
             acc[7] = sq3(cross3(s, H));
             acc[8] = dot3(s, H);
-            int n0;
-            double ratio;
-            interp_index(D.rho[a], T.in_over_rhomax, n0, ratio);
-            double A = lerp_tab(T.rhotab, kRhoStride, H_A, n0, ratio);
-            double B = lerp_tab(T.rhotab, kRhoStride, H_B, n0, ratio);
-            double C = lerp_tab(T.rhotab, kRhoStride, H_C, n0, ratio);
-            double Dd = lerp_tab(T.rhotab, kRhoStride, H_D, n0, ratio);
+            const double4 lan = D.lan[a];
+            const double A = lan.x, B = lan.y, C = lan.z, Dd = lan.w;
             double s2 = sq3(s), s4 = s2 * s2;
             double s0 = sqrt(s2);
             double k = -(2.0 * A + 4.0 * B * s2 + 6.0 * C * s4 + 8.0 * Dd * s4 * s2);
