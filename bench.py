@@ -65,16 +65,19 @@ STEP_FLOP = 14.6e3
 STEP_BYTES = 381.0
 
 
-def traffic_of(kname: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE, each scaled by what the same counter reads for k_dr, a
-    streaming kernel of known traffic; read from the committed summary when it
-    covers this kernel."""
+def traffic_of(kname: str, key: str = "bytes_per_launch"):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (rocprofv3 --pmc, one pass each for FETCH_SIZE, WRITE_SIZE and the L2-to-
+    memory request counts). "bytes_per_launch": 128 B per read request (every
+    request moves one 128-byte line on MI355X; FETCH_SIZE counts 64 B for
+    each, so this is 2 x FETCH_SIZE) plus WRITE_SIZE; "raw_bytes_per_launch":
+    FETCH_SIZE + WRITE_SIZE as rocprofv3 reports them. None when the summary
+    does not cover the kernel."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as fh:
             t = json.load(fh)
-        return t["kernels"][kname]["bytes_per_launch"]
+        return t["kernels"][kname][key]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -381,7 +384,8 @@ def main() -> int:
         dom = max(kern, key=lambda k: kern[k]["share_of_step"] or 0.0)
         r = kern[dom]
         roof = {"bound": r["bound"], "achieved": r["achieved"], "peak": r["peak"], "unit": r["unit"],
-                "frac": r["frac"], "traffic": traffic_of(dom), "kernel": dom,
+                "frac": r["frac"], "traffic": traffic_of(dom),
+                "traffic_raw": traffic_of(dom, "raw_bytes_per_launch"), "kernel": dom,
                 "avg_launch_us": r["avg_launch_us"], "launches_per_step": r["launches_per_step"],
                 "share_of_step": r["share_of_step"]}
     step_roof = {"flop_per_atom_step": STEP_FLOP, "bytes_per_atom_step": STEP_BYTES,
