@@ -248,6 +248,8 @@ def batched_vs_serial(steps: int, warmup: int):
             "serial_atom_steps_per_s": natoms * steps / ts,
             "speedup": ts / tb, "observables_bitwise_equal": equal,
             "untuned_batched_atom_steps_per_s": natoms * steps / t_default,
+            # like for like: the serial contexts run the default shapes too
+            "speedup_default_shapes": ts / t_default,
             "step_graph_batched_atom_steps_per_s": natoms * steps / t_graph,
             "autotune": {"default": default_shapes, "chosen": tuned, "us_per_step": table},
             "workload": "64 bcc Fe replicas, L^3 cubic cells, L uniform in {8..32}, 100-1400 K, "

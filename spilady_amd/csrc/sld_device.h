@@ -41,6 +41,7 @@ __device__ __forceinline__ double3 cross3(double3 a, double3 b) {
 struct Box {
     double d[6];    // xx yx yy zx zy zz
     double inv[6];  // inverse, same layout
+    double safe_sq; // below this squared length a vector's fractional coordinates all lie in (-1/2, 1/2)
     int nx, ny, nz;
 };
 
@@ -64,6 +65,18 @@ __device__ __forceinline__ double3 min_image(const Box& b, double3 rij) {
         rij.x = b.d[0] * q.x + b.d[1] * q.y + b.d[3] * q.z;
         rij.y = b.d[2] * q.y + b.d[4] * q.z;
         rij.z = b.d[5] * q.z;
+    }
+    return rij;
+}
+
+// the minimum image of rij and its squared length: a vector shorter than
+// sqrt(b.safe_sq) needs no shift, so min_image (and its arithmetic) is
+// evaluated only past that length, with the same result
+__device__ __forceinline__ double3 min_image_sq(const Box& b, double3 rij, double& rsq) {
+    rsq = sq3(rij);
+    if (rsq >= b.safe_sq) {
+        rij = min_image(b, rij);
+        rsq = sq3(rij);
     }
     return rij;
 }

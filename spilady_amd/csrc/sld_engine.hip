@@ -313,7 +313,7 @@ int reduce(sld_ctx* c, int nblocks, int nk, double* out, size_t part_offset = 0)
 
 // per-replica sums of block partials into each replica's scalar block
 int reduce_seg(sld_ctx* c, const int* d_seg, int nk, int out_off) {
-    LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)c->d_part, d_seg, nk, c->d_scal, out_off);
+    LAUNCH(c, k_reduce_seg, c->R, 1024, (const double*)c->d_part, d_seg, nk, c->d_scal, out_off);
     return SLD_OK;
 }
 
@@ -626,6 +626,18 @@ int do_check(sld_ctx* c, int step, sld_obs* d_out, bool feedback = true) {
     return SLD_OK;
 }
 
+// squared length below which min_image shifts nothing: fractional coordinate
+// i of r is row i of the inverse box matrix dotted with r, so |q_i| <= |r|
+// times that row's norm; half over the largest norm, less a margin far above
+// the rounding of the products, keeps every |q_i| below 1/2
+double safe_image_sq(const Box& b) {
+    const double n0 = std::sqrt(b.inv[0] * b.inv[0] + b.inv[1] * b.inv[1] + b.inv[3] * b.inv[3]);
+    const double n1 = std::sqrt(b.inv[2] * b.inv[2] + b.inv[4] * b.inv[4]);
+    const double n2 = std::fabs(b.inv[5]);
+    const double r = 0.5 / std::max(n0, std::max(n1, n2)) * (1.0 - 1e-9);
+    return r * r;
+}
+
 // the geometry a box matrix determines (vec_utils_CPU.cpp:135-148,
 // core_dTe_CPU.cpp:143-147): inverse, volume, cell edges; the link-cell
 // counts stay
@@ -643,6 +655,7 @@ void set_box(sld_ctx* c, int r, const double* d) {
     b.inv[3] = (d[1] * d[4] - d[2] * d[3]) / (d[0] * d[2] * d[5]);
     b.inv[4] = -d[4] / (d[2] * d[5]);
     b.inv[5] = 1e0 / d[5];
+    b.safe_sq = safe_image_sq(b);
     h.vol = d[0] * d[2] * d[5];
     const double blx = std::fabs(d[0]);
     const double bly = std::sqrt(d[1] * d[1] + d[2] * d[2]);
@@ -1043,6 +1056,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
         b.inv[3] = (d[1] * d[4] - d[2] * d[3]) / (d[0] * d[2] * d[5]);
         b.inv[4] = -d[4] / (d[2] * d[5]);
         b.inv[5] = 1e0 / d[5];
+        b.safe_sq = safe_image_sq(b);
         h.hc = cell_counts(d, h.prm.min_length_link_cell);
         if (h.hc.nx < 3 || h.hc.ny < 3 || h.hc.nz < 3)
             return set_err(c, SLD_ERR_ARG, "replica %d: need at least 3 link cells per side (got %d %d %d)", r,
@@ -1891,7 +1905,7 @@ int sld_model_eval(sld_ctx* c, double* energy, double* force, double* field) {
     const int nb = nblk(c->D.N, 256);
     LAUNCH(c, k_model_forward, xcd_grid(nb), 256, m.tab, c->phys, c->D, (const double*)m.d_rho, (const double*)m.d_fp,
            (const int*)m.d_coff, m.d_F, m.d_H, m.d_part);
-    LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)m.d_part, (const int*)c->d_atom_seg, 1, m.d_E, 0);
+    LAUNCH(c, k_reduce_seg, c->R, 1024, (const double*)m.d_part, (const int*)c->d_atom_seg, 1, m.d_E, 0);
     if (energy)
         HIPCHK(c, hipMemcpy2DAsync(energy, sizeof(double), m.d_E, kScal * sizeof(double), sizeof(double), c->R,
                                    hipMemcpyDefault, c->stream));
@@ -2007,7 +2021,7 @@ int sld_model_eval_noalloc(sld_ctx* c, double* energy, double* force, double* fi
     double* H = field ? field : m.d_H;
     LAUNCH(c, k_model_forward, xcd_grid(nb), 256, m.tab, c->phys, c->D, (const double*)rho, (const double*)fp,
            (const int*)m.d_coff, F, H, part);
-    LAUNCH(c, k_reduce_seg, c->R, 256, (const double*)part, (const int*)c->d_atom_seg, 1, esum, 0);
+    LAUNCH(c, k_reduce_seg, c->R, 1024, (const double*)part, (const int*)c->d_atom_seg, 1, esum, 0);
     if (energy)
         HIPCHK(c, hipMemcpy2DAsync(energy, sizeof(double), esum, kScal * sizeof(double), sizeof(double), c->R,
                                    hipMemcpyDefault, c->stream));

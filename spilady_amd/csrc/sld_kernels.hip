@@ -16,6 +16,14 @@ namespace sld {
 // ---------------------------------------------------------------------------
 // block reductions into per-block partials (fixed tree order: deterministic)
 
+// sum over the 64 lanes of a wave, by a fixed butterfly of shuffles
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+
 template <int NK>
 __device__ __forceinline__ void block_partials(const double (&v)[NK], double* partials) {
     __shared__ double sh[NK][256];
@@ -32,17 +40,17 @@ __device__ __forceinline__ void block_partials(const double (&v)[NK], double* pa
     }
     if (t == 0) {
 #pragma unroll
-        for (int k = 0; k < NK; ++k) partials[(size_t)blockIdx.x * NK + k] = sh[k][0];
+        for (int k = 0; k < NK; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = sh[k][0];
     }
 }
 
-// out[k] = sum over blocks of partials[b*nk + k]; one block of 256 threads.
+// out[k] = sum over blocks of partials[k*nblk + b]; one block of 256 threads.
 __global__ __launch_bounds__(256) void k_reduce(const double* partials, int nblk, int nk, double* out) {
     __shared__ double sh[256];
     const int t = threadIdx.x;
     for (int k = 0; k < nk; ++k) {
         double acc = 0.0;
-        for (int b = t; b < nblk; b += 256) acc += partials[(size_t)b * nk + k];
+        for (int b = t; b < nblk; b += 256) acc += partials[(size_t)k * nblk + b];
         sh[t] = acc;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
@@ -54,23 +62,47 @@ __global__ __launch_bounds__(256) void k_reduce(const double* partials, int nblk
     }
 }
 
-// Per-replica sums: block r of the grid reduces partial rows [seg[r], seg[r+1])
-// in the same order as k_reduce, into out[r * kScal + out_off + k].
-__global__ __launch_bounds__(256) void k_reduce_seg(const double* partials, const int* seg, int nk, double* out,
-                                                    int out_off) {
-    __shared__ double sh[256];
+// Per-replica sums: block r of the grid (1,024 threads) reduces partial rows
+// [seg[r], seg[r+1]) into out[r * kScal + out_off + k], in an order fixed by
+// the row count alone: thread t adds rows t, t + 1024, ... in turn, each wave
+// sums by shuffles, thread 0 adds the 16 wave sums in order.
+__global__ __launch_bounds__(1024) void k_reduce_seg(const double* partials, const int* seg, int nk, double* out,
+                                                     int out_off) {
+    constexpr int KG = 4;  // components summed together
+    __shared__ double sh[KG][16];
     const int t = threadIdx.x;
     const int b0 = seg[blockIdx.x], nb = seg[blockIdx.x + 1] - b0;
-    for (int k = 0; k < nk; ++k) {
-        double acc = 0.0;
-        for (int b = t; b < nb; b += 256) acc += partials[(size_t)(b0 + b) * nk + k];
-        sh[t] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (t < s) sh[t] += sh[t + s];
-            __syncthreads();
+    const size_t ntot = (size_t)seg[gridDim.x];  // rows of each component
+    for (int k0 = 0; k0 < nk; k0 += KG) {
+        const int kk = min(KG, nk - k0);
+        // thread t adds rows t + 1024 u in order, eight rows of up to four
+        // components in flight
+        double acc[KG] = {0.0, 0.0, 0.0, 0.0};
+        for (int c0 = t; c0 < nb; c0 += 8 * 1024) {
+            double v[8][KG];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int b = c0 + u * 1024;
+#pragma unroll
+                for (int q = 0; q < KG; ++q)
+                    v[u][q] = b < nb && q < kk ? partials[(k0 + q) * ntot + b0 + b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < KG; ++q) acc[q] += v[u][q];
         }
-        if (t == 0) out[(size_t)blockIdx.x * kScal + out_off + k] = sh[0];
+#pragma unroll
+        for (int q = 0; q < KG; ++q) {
+            acc[q] = wave_sum(acc[q]);
+            if ((t & 63) == 0) sh[q][t >> 6] = acc[q];
+        }
+        __syncthreads();
+        if (t < kk) {
+            double a = 0.0;
+            for (int w = 0; w < 16; ++w) a += sh[t][w];
+            out[(size_t)blockIdx.x * kScal + out_off + k0 + t] = a;
+        }
         __syncthreads();
     }
 }
@@ -163,11 +195,6 @@ constexpr int kNbWaves = 4;                 // most waves (cells) per block
 constexpr int kMaxCand = 27 * kCellCap;     // upper bound of candidates
 constexpr int kCandReg = 4;                 // passes of 64 candidates held in registers
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
 
 static_assert(kCellCap <= 64, "k_nbuild holds a cell's own atoms one per lane");
 
@@ -250,8 +277,7 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
             const bool ok = q < npass && jc[q] >= 0 && jc[q] != i;
             double rsq = 1e300;
             if (ok) {
-                const double3 rij = min_image(b, sub3(ri, d3(rc[q])));
-                rsq = sq3(rij);
+                (void)min_image_sq(b, sub3(ri, d3(rc[q])), rsq);
             }
             rsqv[q] = rsq;
             inmaxv[q] = ok && rsq < P.rcut_max_sq;
@@ -306,8 +332,7 @@ __global__ __launch_bounds__(256) void k_nbuild(Tables T, Phys P, Dev D, int wri
             bool ok = j >= 0 && j != i;
             double rsq = 1e300;
             if (ok) {
-                double3 rij = min_image(b, sub3(ri, d3(rj)));
-                rsq = sq3(rij);
+                (void)min_image_sq(b, sub3(ri, d3(rj)), rsq);
             }
             const bool inmax = ok && rsq < P.rcut_max_sq;
             const unsigned long long m1 = __ballot(inmax);
@@ -409,8 +434,7 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
         const double* q[kForceBatch];
 #pragma unroll
         for (int u = 0; u < kForceBatch; ++u) {
-            rij[u] = min_image(b, sub3(ri, d3(rj[u])));
-            rsq[u] = sq3(rij[u]);
+            rij[u] = min_image_sq(b, sub3(ri, d3(rj[u])), rsq[u]);
             r[u] = sqrt(rsq[u]);
             int n0;
             interp_index(r[u], T.in_over_rmax, n0, ratio[u]);

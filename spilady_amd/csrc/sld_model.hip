@@ -23,7 +23,7 @@
 namespace sld {
 
 template <int NK>
-__device__ __forceinline__ void model_partials(const double (&v)[NK], double* partials, int block) {
+__device__ __forceinline__ void model_partials(const double (&v)[NK], double* partials, int block, int nblocks) {
     __shared__ double sh[NK][256];
     const int t = threadIdx.x;
 #pragma unroll
@@ -38,7 +38,7 @@ __device__ __forceinline__ void model_partials(const double (&v)[NK], double* pa
     }
     if (t == 0) {
 #pragma unroll
-        for (int k = 0; k < NK; ++k) partials[(size_t)block * NK + k] = sh[k][0];
+        for (int k = 0; k < NK; ++k) partials[(size_t)k * nblocks + block] = sh[k][0];
     }
 }
 
@@ -58,8 +58,8 @@ __global__ __launch_bounds__(256) void k_model_rho(ModelTab M, Phys P, Dev D, do
     const int nn = D.nbn[i];
     const int* nb = D.nb + (size_t)i * kMaxNb;
     for (int k = 0; k < nn; ++k) {
-        const double3 rij = min_image(b, sub3(ri, d3(D.r[nb[k]])));
-        const double rsq = sq3(rij);
+        double rsq;
+        const double3 rij = min_image_sq(b, sub3(ri, d3(D.r[nb[k]])), rsq);
         if (rsq < P.rcut_pot_sq) {
             int n;
             double ratio;
@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void k_model_forward(ModelTab M, Phys P, Dev D
         const int* nb = D.nb + (size_t)i * kMaxNb;
         for (int k = 0; k < nn; ++k) {
             const int j = nb[k];
-            const double3 rij = min_image(b, sub3(ri, d3(D.r[j])));
-            const double rsq = sq3(rij);
+            double rsq;
+            const double3 rij = min_image_sq(b, sub3(ri, d3(D.r[j])), rsq);
             const double r = sqrt(rsq);
             int n;
             double ratio;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_model_forward(ModelTab M, Phys P, Dev D
         H[3 * c] = h.x; H[3 * c + 1] = h.y; H[3 * c + 2] = h.z;
         acc[0] = e;
     }
-    model_partials<1>(acc, partials, lb);
+    model_partials<1>(acc, partials, lb, (D.N + blockDim.x - 1) / blockDim.x);
 }
 
 // adjoint, first pass: Q_i = sum_j (c_ij + c_ji) f'(r_ij)
@@ -158,8 +158,8 @@ __global__ __launch_bounds__(256) void k_model_q(ModelTab M, Phys P, Dev D, cons
     const int* nb = D.nb + (size_t)i * kMaxNb;
     for (int k = 0; k < nn; ++k) {
         const int j = nb[k];
-        const double3 rij = min_image(b, sub3(ri, d3(D.r[j])));
-        const double rsq = sq3(rij);
+        double rsq;
+        const double3 rij = min_image_sq(b, sub3(ri, d3(D.r[j])), rsq);
         if (rsq >= P.rcut_pot_sq) continue;
         const double r = sqrt(rsq);
         const size_t cj = (size_t)coff[rr] + (j - R.atom0);
@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256) void k_model_adjoint(ModelTab M, Phys P, Dev D
         const int* nb = D.nb + (size_t)i * kMaxNb;
         for (int k = 0; k < nn; ++k) {
             const int j = nb[k];
-            const double3 rij = min_image(b, sub3(ri, d3(D.r[j])));
-            const double rsq = sq3(rij);
+            double rsq;
+            const double3 rij = min_image_sq(b, sub3(ri, d3(D.r[j])), rsq);
             const double r = sqrt(rsq);
             const double cij = -dot3(gf, rij) / r;
             int n;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_model_adjoint(ModelTab M, Phys P, Dev D
         acc[P_LA + 2] += ge * s6 - 6.0 * s4 * gh_si;
         acc[P_LA + 3] += ge * s4 * s4 - 8.0 * s6 * gh_si;
     }
-    model_partials<kNParam>(acc, partials, lb);
+    model_partials<kNParam>(acc, partials, lb, (D.N + blockDim.x - 1) / blockDim.x);
 }
 
 }  // namespace sld
