@@ -1258,6 +1258,15 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     merge_levels(sb, c->reps, c->spin_bwd);
     merge_levels(tf, c->reps, c->dte_fwd);
     merge_levels(tb, c->reps, c->dte_bwd);
+    // A wave of the spin sweep walks as many atoms as its fullest cell holds,
+    // and the cells of one level are independent, so each level lists its
+    // cells by descending starting atom count (stable: neighbouring cells stay
+    // together within a count): waves then hold cells of like length and the
+    // longest start first. The order of a level does not change any result.
+    for (LevelSet* L : {&c->spin_fwd, &c->spin_bwd})
+        for (int l = 0; l < L->nlev(); ++l)
+            std::stable_sort(L->cells.begin() + L->off[l], L->cells.begin() + L->off[l + 1],
+                             [&](int a, int b) { return cn[a] > cn[b]; });
     TRY(upload_levels(c, c->spin_fwd));
     TRY(upload_levels(c, c->spin_bwd));
     TRY(upload_dte_levels(c, c->dte_fwd));

@@ -1306,19 +1306,21 @@ __device__ __forceinline__ void reduce2_block(const double* pa, int na, const do
     __shared__ double sh[2][16];
     const int t = threadIdx.x;
     double a = 0.0, e = 0.0;
-    for (int k0 = t; k0 < na; k0 += 8 * 1024) {
-        double v[8];
+    // both arrays' loads in flight together; each array's terms are added in
+    // the order t, t + 1024, ...
+    const int nmax = na > nb ? na : nb;
+    for (int k0 = t; k0 < nmax; k0 += 8 * 1024) {
+        double v[8], w[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = k0 + u * 1024 < na ? pa[k0 + u * 1024] : 0.0;
+        for (int u = 0; u < 8; ++u) {
+            v[u] = k0 + u * 1024 < na ? pa[k0 + u * 1024] : 0.0;
+            w[u] = k0 + u * 1024 < nb ? pb[k0 + u * 1024] : 0.0;
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a += v[u];
-    }
-    for (int k0 = t; k0 < nb; k0 += 8 * 1024) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = k0 + u * 1024 < nb ? pb[k0 + u * 1024] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) e += v[u];
+        for (int u = 0; u < 8; ++u) {
+            a += v[u];
+            e += w[u];
+        }
     }
     a = wave_sum(a);
     e = wave_sum(e);
