@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_model_rho(ModelTab M, Phys P, Dev D, do
     const int* nb = D.nb + (size_t)i * kMaxNb;
     for (int k = 0; k < nn; ++k) {
         double rsq;
-        const double3 rij = min_image_sq(b, sub3(ri, d3(D.r[nb[k]])), rsq);
+        (void)min_image_sq(b, sub3(ri, d3(D.r[nb[k]])), rsq);
         if (rsq < P.rcut_pot_sq) {
             int n;
             double ratio;
