@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SLD_ABI_VERSION 1
+#define SLD_ABI_VERSION 2
 
 #define SLD_OK 0
 #define SLD_ERR_ARG 1       /* invalid argument or input state */
@@ -61,7 +61,9 @@ typedef struct sld_params {
     int32_t renormalize_energy;/* control.h: renormalizeEnergy (needs eltemp) */
     int32_t changestep;        /* control.h: the time step adapts after every step (scale_step_CPU.cpp:33-97);
                                   needs a single replica */
-    int32_t reserved0;
+    int32_t localvol;          /* control.h: each atom's stress over a local volume from its neighbours
+                                  within rcut_vol (calculate_force_energy_CPU.cpp:80-90,266-283) instead of
+                                  box_volume / natom */
     double displace_limit;     /* A, largest displacement per step (default.h:218) */
     double phi_limit;          /* rad, largest spin rotation per step (default.h:219) */
     int32_t extfield;          /* control.h: per-atom external field (external_field_CPU.cpp), zero
@@ -71,18 +73,22 @@ typedef struct sld_params {
                                   (quantum_noise_CPU.cpp); needs lattlang */
     int32_t msteps_quantum;    /* the noise is redrawn every this many core_dp calls (default.h:223) */
     int32_t nfrequency_quantum;/* frequency intervals; the filter has 2x this many taps (default.h:224) */
-    int32_t reserved1;
+    int32_t magmom;            /* control.h: magnetic moments m = -g s as the state's spin variable
+                                  (core_ds_CPU.cpp:37-45,73-81; the J and Landau tables scaled by powers
+                                  of g, potential_CPU.cpp:125-160) */
+    double rcut_vol;           /* A, the localvol cut-off (default.h:208-211, read_variables.cpp:230) */
 } sld_params;
 
 /* Host view of the per-atom state (struct.h:44-120, con- columns of
  * write_config.cpp:92-131). Any pointer may be NULL in sld_download to skip
  * that field; r, p and s are required by sld_create. */
 typedef struct sld_atoms {
-    double *r, *p, *f, *s;
+    double *r, *p, *f, *s;     /* [natom][3]; with magmom s carries the moments m (read_config.cpp:96-99) */
     double *heff_h, *heff_l;
     double *rho, *pe, *ke, *me, *me0;
     double *stress;            /* [natom][6] */
     double *vir, *sum_jij_sj;
+    double *local_volume;      /* A^3 (write_config.cpp:129); box_volume / natom without localvol */
 } sld_atoms;
 
 /* One printed step: the enr-, tmp- and spn- columns

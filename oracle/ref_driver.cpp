@@ -111,10 +111,18 @@ int ref_init_memory(const char* dir, int n, const double* dbox, const double* r,
     box_length_half = vec_times(0.5, box_length);
     box_volume = d.xx*d.yy*d.zz;
     density = double(natom)/box_volume;
+    // read_config.cpp:133-145: with magmom the input column holds the moments
     for (int i = 0; i < natom; ++i) {
         atom_struct* a = first_atom_ptr + i;
+#ifdef magmom
+        a->m = a->s;
+        a->m0 = vec_length(a->m);
+        a->s = vec_divide(a->m, -el_g);
+        a->s0 = vec_length(a->s);
+#else
         a->m = vec_times(-el_g, a->s);
         a->m0 = vec_length(a->m);
+#endif
     }
     // initial.cpp:58-63: extfield.in / extforce.in of the run directory
 #ifdef extfield
@@ -278,6 +286,19 @@ void ref_get_atoms(double* r, double* p, double* f, double* s, double* heff_h,
     }
 }
 
+// Each atom's local volume (the last column of write_config.cpp:129).
+void ref_get_local_volume(double* lv) {
+    for (int i = 0; i < natom; ++i) lv[i] = (first_atom_ptr + i)->local_volume;
+}
+
+// Each atom's magnetic moment m (struct.h:76), [natom][3].
+void ref_get_m(double* m) {
+    for (int i = 0; i < natom; ++i) {
+        const atom_struct* a = first_atom_ptr + i;
+        m[3*i] = a->m.x; m[3*i+1] = a->m.y; m[3*i+2] = a->m.z;
+    }
+}
+
 // Overwrite the dynamic atom fields (to re-synchronise a run).
 void ref_set_atoms(const double* r, const double* p, const double* f, const double* s,
                    const double* rho) {
@@ -287,6 +308,12 @@ void ref_set_atoms(const double* r, const double* p, const double* f, const doub
         a->p.x = p[3*i]; a->p.y = p[3*i+1]; a->p.z = p[3*i+2];
         a->f.x = f[3*i]; a->f.y = f[3*i+1]; a->f.z = f[3*i+2];
         a->s.x = s[3*i]; a->s.y = s[3*i+1]; a->s.z = s[3*i+2];
+#ifdef magmom
+        // the spin column holds the moments (read_config.cpp:136-139)
+        a->m = a->s;
+        a->m0 = vec_length(a->m);
+        a->s = vec_divide(a->m, -el_g);
+#endif
         a->s0 = vec_length(a->s);
         a->rho = rho[i];
     }

@@ -30,7 +30,7 @@ REF_KEYS = ["OMP_threads", "out_body", "step", "interval_of_print_out",
             "min_length_link_cell", "displace_limit", "phi_limit",
             "interval_of_scale_stress", "stress_xx", "stress_yx", "stress_yy",
             "stress_zx", "stress_zy", "stress_zz", "interval_of_scale_pressure",
-            "pressure", "baro_damping_time", "Msteps_quantum", "Nfrequency_quantum",
+            "pressure", "baro_damping_time", "Msteps_quantum", "Nfrequency_quantum", "rcut_vol",
             "interval_of_vsim", "vsim_prec"]
 
 # Fixed switches of the reference build (shipped control.h).
@@ -66,6 +66,22 @@ def available_qn() -> bool:
     return os.path.exists(REF_LIB_QN)
 
 
+# the same sources built with -Dlocalvol (Makefile.ref OUT=_ref/lv)
+REF_LIB_LV = os.path.join(HERE, "_ref", "lv", "libspilady_ref.so")
+
+
+def available_lv() -> bool:
+    return os.path.exists(REF_LIB_LV)
+
+
+# the same sources built with -Dmagmom (Makefile.ref OUT=_ref/mm)
+REF_LIB_MM = os.path.join(HERE, "_ref", "mm", "libspilady_ref.so")
+
+
+def available_mm() -> bool:
+    return os.path.exists(REF_LIB_MM)
+
+
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
@@ -97,6 +113,7 @@ class Reference:
         L.ref_table_raw.argtypes = [C.c_int, C.c_int]
         L.ref_get_atoms.argtypes = [_dp] * 14 + [_ip]
         L.ref_set_atoms.argtypes = [_dp] * 5
+        L.ref_get_local_volume.argtypes = [_dp]
         L.ref_get_cell_lists.argtypes = [C.c_int, _ip, _ip]
         L.ref_get_groups.argtypes = [_ip, _ip]
         L.ref_get_cells.argtypes = [_dp] * 11
@@ -164,6 +181,16 @@ class Reference:
                                out["cell"].ctypes.data_as(_ip))
         return out
 
+    def local_volume(self) -> np.ndarray:
+        lv = np.zeros(self.natom)
+        self.lib.ref_get_local_volume(_ptr(lv))
+        return lv
+
+    def moments(self) -> np.ndarray:
+        m = np.zeros((self.natom, 3))
+        self.lib.ref_get_m(_ptr(m))
+        return m
+
     def set_atoms(self, r, p, f, s, rho) -> None:
         arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (r, p, f, s, rho)]
         self.lib.ref_set_atoms(*[_ptr(x) for x in arr])
@@ -210,14 +237,16 @@ class Reference:
         self.lib.ref_free()
 
 
-def consistent_start(variables: Dict[str, object], atoms, workdir: Optional[str] = None):
+def consistent_start(variables: Dict[str, object], atoms, workdir: Optional[str] = None,
+                     lib: Optional[str] = None):
     """Fill f, rho, pe, me, me0 and stress of `atoms` with the reference's own
     evaluation (calculate_rho + calculate_force_energy, as initialize() does
     without readconf, initial.cpp:78-83), so that a readconf start is
-    consistent. Returns a new AtomState."""
+    consistent. `lib` picks a variant build (the magmom one reads the spin
+    column as moments). Returns a new AtomState."""
     own = workdir is None
     d = workdir or tempfile.mkdtemp(prefix="sldref_")
-    ref = Reference()
+    ref = Reference(lib)
     ref.init_memory(d, variables, atoms)
     ref.ref_calculate_rho()
     ref.ref_calculate_force_energy()

@@ -47,6 +47,8 @@ struct orc_ctx {
     vec *r, *p, *f, *s, *heff_h, *heff_l, *heff_hc;
     vec *hext, *fext;          /* external field (eV) and force (eV/A), extfield/extforce */
     double *s0, *rho, *pe, *ke, *me, *me0, *vir, *stress, *sum_jij_sj, *local_volume;
+    double *sum_rij_m1, *sum_rij_m2;  /* localvol: sums of 1/r and 1/r^2 within rcut_vol (struct.h:105-108) */
+    vec* m;                           /* magmom: the moments, the state's spin variable (struct.h) */
     int *next, *prev, *cell_old, *cell_new;
     /* cells */
     int *head, *tail, *count, *neigh, *type;
@@ -284,6 +286,18 @@ static void potential_table(orc_ctx* o) {
         o->tab[T_DLB][n] = first_derivative(LandauB_gen, rho);
         o->tab[T_DLC][n] = first_derivative(LandauC_gen, rho);
         o->tab[T_DLD][n] = first_derivative(LandauD_gen, rho);
+        if (o->prm.magmom) { /* potential_CPU.cpp:125-160 */
+            o->tab[T_J][n] *= pow(EL_G, 2);
+            o->tab[T_DJ][n] *= pow(EL_G, 2);
+            o->tab[T_LA][n] *= pow(EL_G, 2);
+            o->tab[T_LB][n] *= pow(EL_G, 4);
+            o->tab[T_LC][n] *= pow(EL_G, 6);
+            o->tab[T_LD][n] *= pow(EL_G, 8);
+            o->tab[T_DLA][n] *= pow(EL_G, 2);
+            o->tab[T_DLB][n] *= pow(EL_G, 4);
+            o->tab[T_DLC][n] *= pow(EL_G, 6);
+            o->tab[T_DLD][n] *= pow(EL_G, 8);
+        }
     }
 }
 
@@ -708,7 +722,7 @@ void orc_calculate_rho(orc_ctx* o) {
         }
 }
 
-/* calculate_force_energy_CPU.cpp:33-216 (SLDHL, no localvol/extfield) */
+/* calculate_force_energy_CPU.cpp:33-216 (SLDHL; localvol's sums at :80-90) */
 static void inner_loop(orc_ctx* o, int a) {
     double si_sq = v_sq(o->s[a]);
     int cc = o->cell_new[a];
@@ -725,6 +739,12 @@ static void inner_loop(orc_ctx* o, int a) {
             double sj_sq = v_sq(o->s[w]);
             if (rsq < o->rcut_max_sq && a != w) {
                 double rij0 = sqrt(rsq);
+                if (o->prm.localvol && rij0 < o->prm.rcut_vol) {
+                    o->sum_rij_m1[a] += 1e0 / rij0;
+                    o->sum_rij_m2[a] += 1e0 / rsq;
+                    o->sum_rij_m1[w] += 1e0 / rij0;
+                    o->sum_rij_m2[w] += 1e0 / rsq;
+                }
                 if (rij0 < o->prm.rcut_pot) {
                     double dsf = TR(o, T_DSF, rij0);
                     double ri = o->rho[a], rj = o->rho[w];
@@ -778,13 +798,26 @@ void orc_calculate_force_energy(orc_ctx* o) {
         o->pe[i] = 0e0; o->vir[i] = 0e0;
         for (int k = 0; k < 6; ++k) o->stress[6 * i + k] = 0e0;
         o->me[i] = 0e0; o->me0[i] = 0e0;
+        o->sum_rij_m1[i] = 0e0; o->sum_rij_m2[i] = 0e0;
     }
     for (int g = 0; g < o->ngroups; ++g)
         for (int j = 0; j < o->group_size[g]; ++j) {
             int c = o->group_members[g * o->max_members + j];
             for (int a = o->head[c]; a != -1; a = o->next[a]) inner_loop(o, a);
         }
-    for (int i = 0; i < N; ++i) o->local_volume[i] = o->box_volume / N;
+    /* calculate_force_energy_CPU.cpp:266-283 */
+    if (o->prm.localvol) {
+        double sum_volume = 0e0;
+        for (int i = 0; i < N; ++i) {
+            double local_radius = 0.5e0 * o->sum_rij_m1[i] / o->sum_rij_m2[i];
+            o->local_volume[i] = 4e0 * PI / 3e0 * pow(local_radius, 3e0);
+            sum_volume += o->local_volume[i];
+        }
+        double volume_factor = o->box_volume / sum_volume;
+        for (int i = 0; i < N; ++i) o->local_volume[i] *= volume_factor;
+    } else {
+        for (int i = 0; i < N; ++i) o->local_volume[i] = o->box_volume / N;
+    }
     double m = o->prm.atmass;
     for (int i = 0; i < N; ++i) {
         double* st = o->stress + 6 * i;
@@ -931,6 +964,11 @@ static void calculate_spin(orc_ctx* o, int a, double dt) {
  * backward (tail to head). Each sweep is one noise pass. */
 void orc_core_ds(orc_ctx* o, double dt) {
     double dt_half = dt / 2e0;
+    if (o->prm.magmom) /* core_ds_CPU.cpp:37-45 */
+        for (int i = 0; i < o->natom; ++i) {
+            o->s[i] = v_divide(o->m[i], -EL_G);
+            o->s0[i] = v_length(o->s[i]);
+        }
     for (int g = 0; g < o->ngroups; ++g) {
         for (int j = 0; j < o->group_size[g]; ++j) {
             int c = o->group_members[g * o->max_members + j];
@@ -945,6 +983,8 @@ void orc_core_ds(orc_ctx* o, double dt) {
         }
     }
     o->noise_call++;
+    if (o->prm.magmom) /* core_ds_CPU.cpp:73-81 */
+        for (int i = 0; i < o->natom; ++i) o->m[i] = v_times(-EL_G, o->s[i]);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1430,6 +1470,16 @@ static void load1(double* dst, const double* src, int n) { if (src) memcpy(dst, 
 static void store1(double* dst, const double* src, int n) { if (dst) memcpy(dst, src, sizeof(double) * n); }
 
 /* initial.cpp:31-106 with readconf (read_config.cpp:110-145 for the box) */
+/* magmom: the caller's spin field holds the moments m; the spins follow as
+ * s = m / (-g) (read_config.cpp:136-139) */
+static void magmom_from_input(orc_ctx* o) {
+    if (!o->prm.magmom) return;
+    for (int i = 0; i < o->natom; ++i) {
+        o->m[i] = o->s[i];
+        o->s[i] = v_divide(o->m[i], -EL_G);
+    }
+}
+
 orc_ctx* orc_create(const sld_params* p, const sld_atoms* in, int compute_initial_forces, int rng_mode) {
     orc_ctx* o = (orc_ctx*)calloc(1, sizeof(orc_ctx));
     o->prm = *p;
@@ -1454,12 +1504,15 @@ orc_ctx* orc_create(const sld_params* p, const sld_atoms* in, int compute_initia
     o->s0 = ALLOCD(N); o->rho = ALLOCD(N); o->pe = ALLOCD(N); o->ke = ALLOCD(N);
     o->me = ALLOCD(N); o->me0 = ALLOCD(N); o->vir = ALLOCD(N); o->stress = ALLOCD(6 * N);
     o->sum_jij_sj = ALLOCD(N); o->local_volume = ALLOCD(N);
+    o->sum_rij_m1 = ALLOCD(N); o->sum_rij_m2 = ALLOCD(N);
     o->next = ALLOCI(N); o->prev = ALLOCI(N); o->cell_old = ALLOCI(N); o->cell_new = ALLOCI(N);
     load3(o->r, in->r, N); load3(o->p, in->p, N); load3(o->f, in->f, N); load3(o->s, in->s, N);
     load3(o->heff_h, in->heff_h, N); load3(o->heff_l, in->heff_l, N);
     load1(o->rho, in->rho, N); load1(o->pe, in->pe, N); load1(o->ke, in->ke, N);
     load1(o->me, in->me, N); load1(o->me0, in->me0, N);
     if (in->stress) memcpy(o->stress, in->stress, sizeof(double) * 6 * N);
+    o->m = ALLOCV(N);
+    magmom_from_input(o);
     for (int i = 0; i < N; ++i) o->s0[i] = v_length(o->s[i]);
 
     map_cells(o);
@@ -1494,7 +1547,8 @@ void orc_destroy(orc_ctx* o) {
     if (!o) return;
     for (int t = 0; t < NTAB; ++t) free(o->tab[t]);
     void* ptrs[] = {o->r, o->p, o->f, o->s, o->heff_h, o->heff_l, o->heff_hc, o->s0, o->rho, o->pe,
-                    o->ke, o->me, o->me0, o->vir, o->stress, o->sum_jij_sj, o->local_volume, o->next,
+                    o->ke, o->me, o->me0, o->vir, o->stress, o->sum_jij_sj, o->local_volume,
+                    o->sum_rij_m1, o->sum_rij_m2, o->m, o->next,
                     o->prev, o->cell_old, o->cell_new, o->head, o->tail, o->count, o->neigh, o->type,
                     o->group_size, o->group_members, o->Te, o->Ee, o->Tl, o->Gel, o->Ges, o->Ts_R,
                     o->Ts_L, o->sum_R_up, o->sum_R_dn, o->sum_L_up, o->sum_L_dn, o->ave_p, o->ave_fluct,
@@ -1513,17 +1567,20 @@ void orc_set_external(orc_ctx* o, const double* hext, const double* fext) {
 
 void orc_download(const orc_ctx* o, sld_atoms* out) {
     int N = o->natom;
-    store3(out->r, o->r, N); store3(out->p, o->p, N); store3(out->f, o->f, N); store3(out->s, o->s, N);
+    store3(out->r, o->r, N); store3(out->p, o->p, N); store3(out->f, o->f, N);
+    store3(out->s, o->prm.magmom ? o->m : o->s, N);
     store3(out->heff_h, o->heff_h, N); store3(out->heff_l, o->heff_l, N);
     store1(out->rho, o->rho, N); store1(out->pe, o->pe, N); store1(out->ke, o->ke, N);
     store1(out->me, o->me, N); store1(out->me0, o->me0, N);
     if (out->stress) memcpy(out->stress, o->stress, sizeof(double) * 6 * N);
     store1(out->vir, o->vir, N); store1(out->sum_jij_sj, o->sum_jij_sj, N);
+    store1(out->local_volume, o->local_volume, N);
 }
 
 void orc_upload(orc_ctx* o, const sld_atoms* in) {
     int N = o->natom;
     load3(o->r, in->r, N); load3(o->p, in->p, N); load3(o->s, in->s, N);
+    if (in->s) magmom_from_input(o);
     load3(o->f, in->f, N); load1(o->rho, in->rho, N);
     for (int i = 0; i < N; ++i) o->s0[i] = v_length(o->s[i]);
 }

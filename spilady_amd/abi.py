@@ -36,7 +36,7 @@ class SldParams(C.Structure):
         ("localcolmot", C.c_int32),
         ("renormalize_energy", C.c_int32),
         ("changestep", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("localvol", C.c_int32),
         ("displace_limit", C.c_double),
         ("phi_limit", C.c_double),
         ("extfield", C.c_int32),
@@ -44,13 +44,14 @@ class SldParams(C.Structure):
         ("quantumnoise", C.c_int32),
         ("msteps_quantum", C.c_int32),
         ("nfrequency_quantum", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("magmom", C.c_int32),
+        ("rcut_vol", C.c_double),
     ]
 
 
 class SldAtoms(C.Structure):
     _fields_ = [(k, _dp) for k in ("r", "p", "f", "s", "heff_h", "heff_l", "rho", "pe", "ke",
-                                   "me", "me0", "stress", "vir", "sum_jij_sj")]
+                                   "me", "me0", "stress", "vir", "sum_jij_sj", "local_volume")]
 
 
 class SldObs(C.Structure):
@@ -106,7 +107,7 @@ def barostat_from_variables(v: Dict[str, object]) -> SldBarostat:
 
 
 VEC_FIELDS = ("r", "p", "f", "s", "heff_h", "heff_l")
-SCALAR_FIELDS = ("rho", "pe", "ke", "me", "me0", "vir", "sum_jij_sj")
+SCALAR_FIELDS = ("rho", "pe", "ke", "me", "me0", "vir", "sum_jij_sj", "local_volume")
 CELL_FIELDS = ("te", "ee", "tl", "ts_r", "ts_l", "gel", "ges", "sum_r_up", "sum_r_dn",
                "sum_l_up", "sum_l_dn")
 
@@ -219,6 +220,10 @@ def params_from_variables(v: Dict[str, object], box, natom: int, replica: int = 
     p.quantumnoise = int(v.get("quantumnoise", 0))
     p.msteps_quantum = int(v.get("Msteps_quantum", 50))
     p.nfrequency_quantum = int(v.get("Nfrequency_quantum", 150))
+    # localvol and magmom (control.h); rcut_vol defaults to 1.2 a_lattice (default.h:96,208-211)
+    p.localvol = int(v.get("localvol", 0))
+    p.rcut_vol = float(v.get("rcut_vol", 2.83 * 1.2))
+    p.magmom = int(v.get("magmom", 0))
     return p
 
 

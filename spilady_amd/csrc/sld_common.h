@@ -8,6 +8,7 @@ constexpr double kBoltz = 8.617343e-5;       // eV/K
 constexpr double kHbar = 6.58211899e-16;     // eV s
 constexpr double kElG = 2.0023193043622e0;   // electron g factor
 constexpr double kEvToJ = 1.602176487e-19;
+constexpr double kPi = 3.141592653589793238462e0;  // Pi_num (spilady.h:61)
 
 // potential tables (global.h:66): 10^6 points per table
 constexpr int kNInput = 1000000;

@@ -359,12 +359,20 @@ int do_force(sld_ctx* c) {
     LAUNCH(c, k_fp, xcd_grid(nblk(c->D.N, 256)), 256, c->tab, c->D);
     LAUNCH(c, k_force, xcd_grid(nblk(c->D.N, c->shapes.force_block)), c->shapes.force_block, c->tab, c->phys,
            c->D);
+    if (c->phys.localvol) {
+        // calculate_force_energy_CPU.cpp:270-293: the volumes normalised per
+        // replica, then the stress over them
+        LAUNCH(c, k_lvol_sum, nblk(c->D.N, 256), 256, c->D, c->d_part);
+        TRY(reduce_seg(c, c->d_atom_seg, 1, kScalLvol));
+        LAUNCH(c, k_lvol_apply, nblk(c->D.N, 256), 256, c->D, (const double*)c->d_scal);
+    }
     return SLD_OK;
 }
 
 int do_ds(sld_ctx* c, double dt) {
     TRY(ensure_nb(c));
     double dth = dt / 2.0;
+    if (c->phys.magmom) LAUNCH(c, k_magmom_s, nblk(c->D.N, 256), 256, c->D, 0, c->D.N);
     for (int dir = 0; dir < 2; ++dir) {
         LevelSet& L = dir == 0 ? c->spin_fwd : c->spin_bwd;
         if (c->phys.spinlang)
@@ -383,6 +391,7 @@ int do_ds(sld_ctx* c, double dt) {
         }
         c->pass++;
     }
+    if (c->phys.magmom) LAUNCH(c, k_magmom_m, nblk(c->D.N, 256), 256, c->D);
     return SLD_OK;
 }
 
@@ -870,6 +879,18 @@ int up4(sld_ctx* c, double4* d, const double* src, int r, bool with_len) {
     return SLD_OK;
 }
 
+// The caller's spin field of a replica: the spins, or with magmom the moments
+// m, from which the spins follow as s = m / (-g) (read_config.cpp:136-139)
+int up_spins(sld_ctx* c, const double* src, int r) {
+    if (!c->phys.magmom) return up4(c, c->D.s, src, r, true);
+    if (!src) return SLD_OK;
+    TRY(up4(c, c->D.m, src, r, false));
+    const Rep& R = c->reps[r];
+    LAUNCH(c, k_magmom_s, nblk(R.natom, 256), 256, c->D, R.atom0, R.natom);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SLD_OK;
+}
+
 // electron energy of a cell at Te (eV), in the arithmetic of heatcapacity_CPU.cpp:42-48
 double host_te_to_ee(double te) {
     const double bb = 2e-4 / kBoltz;
@@ -969,6 +990,7 @@ bool same_shared(const sld_params& a, const sld_params& b) {
            a.lattlang == b.lattlang && a.eltemp == b.eltemp && a.localcolmot == b.localcolmot &&
            a.renormalize_energy == b.renormalize_energy && a.changestep == b.changestep &&
            a.extfield == b.extfield && a.extforce == b.extforce && a.quantumnoise == b.quantumnoise &&
+           a.localvol == b.localvol && (!a.localvol || a.rcut_vol == b.rcut_vol) && a.magmom == b.magmom &&
            (!a.quantumnoise ||
             (a.msteps_quantum == b.msteps_quantum && a.nfrequency_quantum == b.nfrequency_quantum));
 }
@@ -1029,6 +1051,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
         return set_err(c, SLD_ERR_ARG, "the electron-temperature model runs on single-system contexts only");
     if (nrep > 1 && P0.changestep)
         return set_err(c, SLD_ERR_ARG, "changestep runs on single-system contexts only (one time step per context)");
+    if (P0.localvol && !(P0.rcut_vol > 0.0)) return set_err(c, SLD_ERR_ARG, "localvol needs rcut_vol > 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_err(c, SLD_ERR_NOGPU, "no HIP device");
@@ -1103,9 +1126,12 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     ph.quantumnoise = P0.quantumnoise;
     ph.qm = P0.msteps_quantum;
     ph.qnf2 = 2 * P0.nfrequency_quantum;
+    ph.localvol = P0.localvol;
+    ph.magmom = P0.magmom;
+    ph.rcut_vol = P0.rcut_vol;
 
     // tables (shared)
-    std::shared_ptr<const HostTables> ht = shared_tables(P0.rcut_max);
+    std::shared_ptr<const HostTables> ht = shared_tables(P0.rcut_max, P0.magmom != 0);
     TRY(upload_vec(c, &c->d_prow, ht->prow));
     TRY(upload_vec(c, &c->d_rhotab, ht->rhotab));
     c->tab.prow = c->d_prow;
@@ -1159,6 +1185,8 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
     }
     TRY(dalloc(c, &D.pass_base, 1));
     TRY(dalloc(c, &D.sjs, N)); TRY(dalloc(c, &D.fp, N)); TRY(dalloc(c, &D.stress, (size_t)6 * N));
+    TRY(dalloc(c, &D.lvol, P0.localvol ? N : 1));
+    TRY(dalloc(c, &D.m, P0.magmom ? N : 1));
     TRY(dalloc(c, &D.cell, N)); TRY(dalloc(c, &D.cell_prev, N)); TRY(dalloc(c, &D.cslot, N));
     TRY(dalloc(c, &D.cl, (size_t)C * kCellCap)); TRY(dalloc(c, &D.cn, C));
     TRY(dalloc(c, &D.inc, (size_t)C * kCellCap)); TRY(dalloc(c, &D.inc_n, C));
@@ -1215,7 +1243,7 @@ int create_impl(sld_ctx* c, int nrep, const sld_params* params, const sld_atoms*
         TRY(up4(c, D.r, r3.data(), r, false));
         TRY(up4(c, D.p, in->p, r, false));
         TRY(up4(c, D.f, in->f, r, false));
-        TRY(up4(c, D.s, in->s, r, true));
+        TRY(up_spins(c, in->s, r));
         TRY(up4(c, D.hh, in->heff_h, r, false));
         TRY(up4(c, D.hl, in->heff_l, r, false));
         TRY(up1(c, D.rho, in->rho, R.atom0, n));
@@ -1498,10 +1526,19 @@ int sld_download(sld_ctx* c, sld_atoms* o) {
     const Rep& R = c->reps[r];
     const size_t a0 = R.atom0, n = R.natom;
     TRY(down4(c, D.r, o->r, r)); TRY(down4(c, D.p, o->p, r)); TRY(down4(c, D.f, o->f, r));
-    TRY(down4(c, D.s, o->s, r)); TRY(down4(c, D.hh, o->heff_h, r)); TRY(down4(c, D.hl, o->heff_l, r));
+    TRY(down4(c, c->phys.magmom ? D.m : D.s, o->s, r));
+    TRY(down4(c, D.hh, o->heff_h, r)); TRY(down4(c, D.hl, o->heff_l, r));
     TRY(down1(c, D.rho, o->rho, a0, n)); TRY(down1(c, D.pe, o->pe, a0, n)); TRY(down1(c, D.ke, o->ke, a0, n));
     TRY(down1(c, D.me, o->me, a0, n)); TRY(down1(c, D.me0, o->me0, a0, n)); TRY(down1(c, D.vir, o->vir, a0, n));
     TRY(down1(c, D.sjs, o->sum_jij_sj, a0, n));
+    if (o->local_volume) {
+        if (c->phys.localvol) {
+            TRY(down1(c, D.lvol, o->local_volume, a0, n));
+        } else {
+            // calculate_force_energy_CPU.cpp:276
+            for (size_t i = 0; i < n; ++i) o->local_volume[i] = R.box_volume / R.natom;
+        }
+    }
     if (o->stress) {
         std::vector<double> st(n);
         for (int k = 0; k < 6; ++k) {
@@ -1520,7 +1557,7 @@ int sld_upload(sld_ctx* c, const sld_atoms* in) {
     const Rep& R = c->reps[r];
     TRY(up4(c, D.r, in->r, r, false));
     TRY(up4(c, D.p, in->p, r, false));
-    TRY(up4(c, D.s, in->s, r, true));
+    TRY(up_spins(c, in->s, r));
     TRY(up4(c, D.f, in->f, r, false));
     TRY(up1(c, D.rho, in->rho, R.atom0, R.natom));
     TRY(refresh_landau(c));

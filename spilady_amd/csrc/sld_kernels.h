@@ -44,6 +44,8 @@ struct Dev {
     double *rho, *pe, *ke, *me, *me0, *vir, *sjs, *fp;
     double4* lan;              // Landau A, B, C, D at each atom's density [N] (kept current with rho)
     double* stress;            // [6][N]: 11 22 33 12 23 31
+    double* lvol;              // localvol: each atom's local volume
+    double4* m;                // magmom: the moments m = -g s, the state between spin passes
     int* cell;                 // link cell of each atom (new_cell_index)
     int* cell_prev;            // link cell before the last links()
     int* cslot;                // cell * kCellCap + position in the cell's list
@@ -74,10 +76,14 @@ struct Phys {
     int spinlang, lattlang, eltemp, localcolmot;
     int extfield, extforce;
     int quantumnoise, qm, qnf2;  // quantum thermostat: on, redraw period M, filter taps 2 Nf
+    int localvol;                // stress over per-atom local volumes
+    int magmom;                  // moments as the spin variable
+    double rcut_vol;             // their cut-off, A
 };
 
 // per-replica block of reduced scalars
 constexpr int kScal = 64;
+constexpr int kScalLvol = 56;  // localvol: the replica's sum of local volumes
 // lanes per link cell in the momentum and temperature passes (a lane takes
 // the cell's list entries l, l + kSeg, ...)
 constexpr int kSeg = 8;
@@ -88,6 +94,10 @@ __global__ void k_relink(const int* cl_old, const int* cn_old, Dev D);
 __global__ void k_nbuild(Tables T, Phys P, Dev D, int write_rho);
 __global__ void k_fp(Tables T, Dev D);
 __global__ void k_force(Tables T, Phys P, Dev D);
+__global__ void k_lvol_sum(Dev D, double* partials);
+__global__ void k_lvol_apply(Dev D, const double* scal);
+__global__ void k_magmom_s(Dev D, int a0, int n);
+__global__ void k_magmom_m(Dev D);
 __global__ void k_spin_noise(Dev D, int pass_off);
 __global__ void k_landau(Tables T, Dev D);
 __global__ void k_step_max(Dev D, double* partials);

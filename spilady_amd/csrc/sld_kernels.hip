@@ -410,6 +410,7 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     double3 f = make_double3(0.0, 0.0, 0.0);
     double pe = 0.0, me = 0.0, me0 = 0.0, vir = 0.0;
     double st0 = 0.0, st1 = 0.0, st2 = 0.0, st3 = 0.0, st4 = 0.0, st5 = 0.0;
+    double m1 = 0.0, m2 = 0.0;  // localvol: sums of 1/r and 1/r^2 within rcut_vol
     const int nn = D.nbn[i];
     const int* nb = D.nb + (size_t)i * kMaxNb;
     // kForceBatch pairs at a time: their neighbour ids, then their
@@ -470,6 +471,10 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
             st0 += fij.x * rij[u].x; st1 += fij.y * rij[u].y; st2 += fij.z * rij[u].z;
             st3 += fij.x * rij[u].y; st4 += fij.y * rij[u].z; st5 += fij.z * rij[u].x;
             vir += -force * rsq[u];
+            if (P.localvol && r[u] < P.rcut_vol) {  // calculate_force_energy_CPU.cpp:80-90
+                m1 += 1.0 / r[u];
+                m2 += 1.0 / rsq[u];
+            }
         }
     }
     // one-body terms (calculate_force_energy_CPU.cpp:196-215)
@@ -490,17 +495,48 @@ __global__ __launch_bounds__(256) void k_force(Tables T, Phys P, Dev D) {
     // the reference books -force*r^2 once per pair on the half-shell owner;
     // here each atom keeps half of each of its pairs (same total virial)
     D.vir[i] = 0.5 * vir;
-    // per-atom stress with the kinetic part (calculate_force_energy_CPU.cpp:283-293)
-    const double lv = R.box_volume / R.natom;
+    // per-atom stress with the kinetic part (calculate_force_energy_CPU.cpp:283-293);
+    // with localvol the local volume (:270-276) is normalised over the
+    // replica first, so the numerators are stored and k_lvol_apply divides
     const double m = P.atmass;
     const double4 p = D.p[i];
     const size_t N = D.N;
-    D.stress[0 * N + i] = (p.x * p.x / m + st0 / 2.0) / lv;
-    D.stress[1 * N + i] = (p.y * p.y / m + st1 / 2.0) / lv;
-    D.stress[2 * N + i] = (p.z * p.z / m + st2 / 2.0) / lv;
-    D.stress[3 * N + i] = ((p.x * p.y) / m + st3 / 2.0) / lv;
-    D.stress[4 * N + i] = ((p.y * p.z) / m + st4 / 2.0) / lv;
-    D.stress[5 * N + i] = ((p.z * p.x) / m + st5 / 2.0) / lv;
+    double num[6] = {p.x * p.x / m + st0 / 2.0,   p.y * p.y / m + st1 / 2.0,   p.z * p.z / m + st2 / 2.0,
+                     (p.x * p.y) / m + st3 / 2.0, (p.y * p.z) / m + st4 / 2.0, (p.z * p.x) / m + st5 / 2.0};
+    if (P.localvol) {
+        const double local_radius = 0.5 * m1 / m2;
+        D.lvol[i] = 4.0 * kPi / 3.0 * pow(local_radius, 3.0);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) D.stress[k * N + i] = num[k];
+    } else {
+        const double lv = R.box_volume / R.natom;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) D.stress[k * N + i] = num[k] / lv;
+    }
+}
+
+// localvol: each replica's sum of the local volumes, one partial per 256
+// atom slots (calculate_force_energy_CPU.cpp:270-275)
+__global__ __launch_bounds__(256) void k_lvol_sum(Dev D, double* partials) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (a < D.N && D.arep[a] >= 0) acc[0] = D.lvol[a];
+    block_partials<1>(acc, partials);
+}
+
+// localvol: the volumes scaled to fill the box, then each atom's stress over
+// its volume (calculate_force_energy_CPU.cpp:279-293)
+__global__ __launch_bounds__(256) void k_lvol_apply(Dev D, const double* scal) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.N) return;
+    const int rr = D.arep[a];
+    if (rr < 0) return;
+    const double volume_factor = D.rep[rr].box_volume / scal[(size_t)rr * kScal + kScalLvol];
+    const double lv = D.lvol[a] * volume_factor;
+    D.lvol[a] = lv;
+    const size_t N = D.N;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) D.stress[k * N + a] = D.stress[k * N + a] / lv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1584,6 +1620,25 @@ __global__ __launch_bounds__(256) void k_spin_len(Dev D, int a0, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     D.s[a0 + k].w = spin_len(d3(D.s[a0 + k]));
+}
+
+// magmom: the spins from the moments, s = m / (-g), with |s| in .w
+// (core_ds_CPU.cpp:37-45, read_config.cpp:136-139), for atom slots
+// [a0, a0 + n)
+__global__ __launch_bounds__(256) void k_magmom_s(Dev D, int a0, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double4 m = D.m[a0 + k];
+    const double3 s = make_double3(m.x / -kElG, m.y / -kElG, m.z / -kElG);
+    D.s[a0 + k] = make_double4(s.x, s.y, s.z, spin_len(s));
+}
+
+// magmom: the moments from the spins, m = -g s (core_ds_CPU.cpp:73-81)
+__global__ __launch_bounds__(256) void k_magmom_m(Dev D) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.N) return;
+    const double4 s = D.s[a];
+    D.m[a] = make_double4(-kElG * s.x, -kElG * s.y, -kElG * s.z, 0.0);
 }
 
 __global__ __launch_bounds__(256) void k_check_nonfinite(Dev D) {

@@ -167,7 +167,7 @@ double deriv6(F fn, double x) {
 
 }  // namespace
 
-void build_tables(double rcut_max, HostTables& t) {
+void build_tables(double rcut_max, bool magmom, HostTables& t) {
     t.rmax = rcut_max;
     t.rhomax = 2e0 * smallf_gen(1e0);
     if (t.rhomax < 1000e0) t.rhomax = 1000e0;
@@ -199,6 +199,18 @@ void build_tables(double rcut_max, HostTables& t) {
             H[H_DB] = deriv6(landauB, rho);
             H[H_DC] = deriv6(landauC, rho);
             H[H_DD] = deriv6(landauD, rho);
+            if (magmom) {
+                R[R_J] *= std::pow(kElG, 2);
+                R[R_DJ] *= std::pow(kElG, 2);
+                H[H_A] *= std::pow(kElG, 2);
+                H[H_B] *= std::pow(kElG, 4);
+                H[H_C] *= std::pow(kElG, 6);
+                H[H_D] *= std::pow(kElG, 8);
+                H[H_DA] *= std::pow(kElG, 2);
+                H[H_DB] *= std::pow(kElG, 4);
+                H[H_DC] *= std::pow(kElG, 6);
+                H[H_DD] *= std::pow(kElG, 8);
+            }
         }
     };
     unsigned nt = std::thread::hardware_concurrency();
@@ -223,15 +235,20 @@ void build_tables(double rcut_max, HostTables& t) {
     }
 }
 
-std::shared_ptr<const HostTables> shared_tables(double rcut_max) {
+std::shared_ptr<const HostTables> shared_tables(double rcut_max, bool magmom) {
     static std::mutex mu;
-    static std::vector<std::pair<double, std::shared_ptr<const HostTables>>> cache;
+    struct Entry {
+        double rcut_max;
+        bool magmom;
+        std::shared_ptr<const HostTables> t;
+    };
+    static std::vector<Entry> cache;
     std::lock_guard<std::mutex> lock(mu);
     for (auto& e : cache)
-        if (e.first == rcut_max) return e.second;
+        if (e.rcut_max == rcut_max && e.magmom == magmom) return e.t;
     auto t = std::make_shared<HostTables>();
-    build_tables(rcut_max, *t);
-    cache.emplace_back(rcut_max, t);
+    build_tables(rcut_max, magmom, *t);
+    cache.push_back({rcut_max, magmom, t});
     return t;
 }
 

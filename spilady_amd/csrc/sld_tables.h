@@ -13,7 +13,9 @@ struct HostTables {
     std::vector<double> rhotab;  // [kNInput][kRhoStride]
 };
 
-void build_tables(double rcut_max, HostTables& t);
+// magmom: the J and Landau columns scaled by powers of the electron g factor
+// (potential_CPU.cpp:125-160)
+void build_tables(double rcut_max, bool magmom, HostTables& t);
 
 // pieces of the DD05 pair function shared with the parametric model
 double zbl_pair(double r);                 // DD05.cpp:45-62
@@ -25,8 +27,8 @@ struct Bridge {
 };
 ZblEnds zbl_ends();
 Bridge bridge_from_ends(const ZblEnds& e, double y2, double Dy2, double D2y2);
-// The tables for rcut_max, built once per process and shared by every
-// context created with the same cutoff.
-std::shared_ptr<const HostTables> shared_tables(double rcut_max);
+// The tables for rcut_max (and magmom), built once per process and shared by
+// every context created with the same cutoff.
+std::shared_ptr<const HostTables> shared_tables(double rcut_max, bool magmom);
 
 }  // namespace sld

@@ -42,6 +42,8 @@
 #pragma push_macro("extfield")
 #pragma push_macro("extforce")
 #pragma push_macro("quantumnoise")
+#pragma push_macro("localvol")
+#pragma push_macro("magmom")
 #undef spinlang
 #undef lattlang
 #undef eltemp
@@ -50,6 +52,8 @@
 #undef extfield
 #undef extforce
 #undef quantumnoise
+#undef localvol
+#undef magmom
 #include "spilady_hip.h"
 
 namespace {
@@ -57,7 +61,7 @@ namespace {
 // control.h switches, in this order, into sld_params; defined here, where
 // the field names are not yet macros
 enum { kSpinlang, kLattlang, kEltemp, kLocalcolmot, kRenormalize, kChangestep, kExtfield, kExtforce,
-       kQuantumnoise, kNSwitch };
+       kQuantumnoise, kLocalvol, kMagmom, kNSwitch };
 void apply_switches(sld_params& p, const int (&on)[kNSwitch]) {
     p.spinlang = on[kSpinlang];
     p.lattlang = on[kLattlang];
@@ -68,6 +72,8 @@ void apply_switches(sld_params& p, const int (&on)[kNSwitch]) {
     p.extfield = on[kExtfield];
     p.extforce = on[kExtforce];
     p.quantumnoise = on[kQuantumnoise];
+    p.localvol = on[kLocalvol];
+    p.magmom = on[kMagmom];
 }
 
 }  // namespace
@@ -80,6 +86,8 @@ void apply_switches(sld_params& p, const int (&on)[kNSwitch]) {
 #pragma pop_macro("extfield")
 #pragma pop_macro("extforce")
 #pragma pop_macro("quantumnoise")
+#pragma pop_macro("localvol")
+#pragma pop_macro("magmom")
 
 #include "spilady.h"
 
@@ -100,22 +108,22 @@ void ok(int rc, const char* what) {
 // atom_struct array (struct.h:44-120).
 struct Soa {
     double* mem;
-    double *r, *p, *f, *s, *hh, *hl, *rho, *pe, *ke, *me, *me0, *stress, *vir, *sjs;
+    double *r, *p, *f, *s, *hh, *hl, *rho, *pe, *ke, *me, *me0, *stress, *vir, *sjs, *lv;
     explicit Soa(int n) {
         const size_t m = (size_t)n;
-        mem = new double[m * (6 * 3 + 6 + 7)]();
+        mem = new double[m * (6 * 3 + 6 + 8)]();
         double* q = mem;
         double** vec3[] = {&r, &p, &f, &s, &hh, &hl};
         for (double** x : vec3) { *x = q; q += 3 * m; }
         stress = q; q += 6 * m;
-        double** sc[] = {&rho, &pe, &ke, &me, &me0, &vir, &sjs};
+        double** sc[] = {&rho, &pe, &ke, &me, &me0, &vir, &sjs, &lv};
         for (double** x : sc) { *x = q; q += m; }
     }
     ~Soa() { delete[] mem; }
     Soa(const Soa&) = delete;
     Soa& operator=(const Soa&) = delete;
     sld_atoms view() const {
-        sld_atoms v = {r, p, f, s, hh, hl, rho, pe, ke, me, me0, stress, vir, sjs};
+        sld_atoms v = {r, p, f, s, hh, hl, rho, pe, ke, me, me0, stress, vir, sjs, lv};
         return v;
     }
 };
@@ -126,10 +134,15 @@ void put3(double* d, int i, const ::vector& x) { d[3 * i] = x.x; d[3 * i + 1] = 
 void gather(Soa& a) {
     for (int i = 0; i < natom; ++i) {
         const atom_struct* t = first_atom_ptr + i;
-        put3(a.r, i, t->r); put3(a.p, i, t->p); put3(a.f, i, t->f); put3(a.s, i, t->s);
+        put3(a.r, i, t->r); put3(a.p, i, t->p); put3(a.f, i, t->f);
+#ifdef magmom
+        put3(a.s, i, t->m);  // with magmom the library's spin field carries the moments
+#else
+        put3(a.s, i, t->s);
+#endif
         put3(a.hh, i, t->Heff_H); put3(a.hl, i, t->Heff_L);
         a.rho[i] = t->rho; a.pe[i] = t->pe; a.ke[i] = t->ke; a.me[i] = t->me; a.me0[i] = t->me0;
-        a.vir[i] = t->vir; a.sjs[i] = t->sum_Jij_sj;
+        a.vir[i] = t->vir; a.sjs[i] = t->sum_Jij_sj; a.lv[i] = t->local_volume;
         const double st[6] = {t->stress11, t->stress22, t->stress33, t->stress12, t->stress23, t->stress31};
         for (int k = 0; k < 6; ++k) a.stress[6 * i + k] = st[k];
     }
@@ -138,19 +151,26 @@ void gather(Soa& a) {
 void scatter(const Soa& a) {
     for (int i = 0; i < natom; ++i) {
         atom_struct* t = first_atom_ptr + i;
-        t->r = get3(a.r, i); t->p = get3(a.p, i); t->f = get3(a.f, i); t->s = get3(a.s, i);
+        t->r = get3(a.r, i); t->p = get3(a.p, i); t->f = get3(a.f, i);
         t->Heff_H = get3(a.hh, i); t->Heff_L = get3(a.hl, i);
         t->rho = a.rho[i]; t->pe = a.pe[i]; t->ke = a.ke[i]; t->me = a.me[i]; t->me0 = a.me0[i];
         t->vir = a.vir[i]; t->sum_Jij_sj = a.sjs[i];
+#ifdef magmom
+        // core_ds_CPU.cpp:37-45,73-81: the moments are the state, the spins follow
+        t->m = get3(a.s, i);
+        t->m0 = vec_length(t->m);
+        t->s = vec_divide(t->m, -el_g);
+        t->s0 = vec_length(t->s);
+#else
+        t->s = get3(a.s, i);
         t->s0 = vec_length(t->s);
         t->m = vec_times(-el_g, t->s);
         t->m0 = el_g * t->s0;
+#endif
         t->stress11 = a.stress[6 * i]; t->stress22 = a.stress[6 * i + 1]; t->stress33 = a.stress[6 * i + 2];
         t->stress12 = a.stress[6 * i + 3]; t->stress23 = a.stress[6 * i + 4]; t->stress31 = a.stress[6 * i + 5];
-#ifndef localvol
-        // the per-atom volume the force pass sets (calculate_force_energy_CPU.cpp:276)
-        t->local_volume = box_volume / double(natom);
-#endif
+        // the per-atom volume of the last force pass (calculate_force_energy_CPU.cpp:266-283)
+        t->local_volume = a.lv[i];
     }
 }
 
@@ -247,6 +267,13 @@ void initial_GPU() {
     on[kQuantumnoise] = 1;
     prm.msteps_quantum = Msteps_quantum;
     prm.nfrequency_quantum = Nfrequency_quantum;
+#endif
+#ifdef localvol
+    on[kLocalvol] = 1;
+    prm.rcut_vol = rcut_vol;
+#endif
+#ifdef magmom
+    on[kMagmom] = 1;
 #endif
     apply_switches(prm, on);
     Soa a(natom);

@@ -54,6 +54,8 @@ struct Vars {
         {"renormalizeEnergy", "1"}, {"runstep", "1"}, {"changestep", "0"},
         {"total_production_time", "1e-12"}, {"displace_limit", "0.01"}, {"phi_limit", "0.628319"},
         {"extfield", "0"}, {"extforce", "0"},
+        // control.h localvol with its cut-off 1.2 a_lattice (default.h:96,208-211), and magmom
+        {"localvol", "0"}, {"rcut_vol", "3.396"}, {"magmom", "0"},
         // Berendsen barostats: control.h STRESS / PRESSURE, default.h:177-193
         {"STRESS", "0"}, {"PRESSURE", "0"}, {"interval_of_scale_stress", "1"}, {"stress_xx", "0"},
         {"stress_yx", "0"}, {"stress_yy", "0"}, {"stress_zx", "0"}, {"stress_zy", "0"}, {"stress_zz", "0"},
@@ -80,18 +82,18 @@ struct Atoms {
     int n = 0;
     double total_time = 0.0;
     double box[6] = {0, 0, 0, 0, 0, 0};
-    std::vector<double> r, p, f, s, hh, hl, stress, rho, pe, ke, me, me0, vir, sjs;
+    std::vector<double> r, p, f, s, hh, hl, stress, rho, pe, ke, me, me0, vir, sjs, lv;
     std::vector<std::string> el;  // element column of the con- file
     void resize(int natom) {
         n = natom;
         el.assign(n, "Fe");
         for (auto* v : {&r, &p, &f, &s, &hh, &hl}) v->assign((size_t)3 * n, 0.0);
-        for (auto* v : {&rho, &pe, &ke, &me, &me0, &vir, &sjs}) v->assign(n, 0.0);
+        for (auto* v : {&rho, &pe, &ke, &me, &me0, &vir, &sjs, &lv}) v->assign(n, 0.0);
         stress.assign((size_t)6 * n, 0.0);
     }
     sld_atoms view() {
         return {r.data(), p.data(), f.data(), s.data(), hh.data(), hl.data(), rho.data(), pe.data(),
-                ke.data(), me.data(), me0.data(), stress.data(), vir.data(), sjs.data()};
+                ke.data(), me.data(), me0.data(), stress.data(), vir.data(), sjs.data(), lv.data()};
     }
 };
 
@@ -109,7 +111,7 @@ Atoms read_con(const std::string& path) {
     a.resize(n);
     for (int i = 0; i < n; ++i) {
         int idx;
-        double s0, lv, hx, hy, hz, lx, ly, lz;
+        double s0, hx, hy, hz, lx, ly, lz;
         in >> idx >> a.el[i];
         for (int k = 0; k < 3; ++k) in >> a.r[3 * i + k];
         for (int k = 0; k < 3; ++k) in >> a.p[3 * i + k];
@@ -117,7 +119,7 @@ Atoms read_con(const std::string& path) {
         for (int k = 0; k < 6; ++k) in >> a.stress[6 * i + k];
         in >> a.rho[i] >> a.ke[i] >> a.pe[i];
         for (int k = 0; k < 3; ++k) in >> a.s[3 * i + k];
-        in >> s0 >> hx >> hy >> hz >> lx >> ly >> lz >> a.me[i] >> a.me0[i] >> lv;
+        in >> s0 >> hx >> hy >> hz >> lx >> ly >> lz >> a.me[i] >> a.me0[i] >> a.lv[i];
         // the y columns repeat x (write_config.cpp:117,120); kept as read
         a.hh[3 * i] = hx; a.hh[3 * i + 1] = hy; a.hh[3 * i + 2] = hz;
         a.hl[3 * i] = lx; a.hl[3 * i + 1] = ly; a.hl[3 * i + 2] = lz;
@@ -181,8 +183,8 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
     std::fprintf(f, "%.15e %.15e %.15e\n%.15e %.15e %.15e\n", a.box[0], a.box[1], a.box[2], a.box[3], a.box[4],
                  a.box[5]);
     // local_volume: box_volume / natom (calculate_force_energy_CPU.cpp:276;
-    // box_volume = d.xx d.yy d.zz, read_config.cpp:126)
-    const double lv = a.box[0] * a.box[2] * a.box[5] / a.n;
+    // box_volume = d.xx d.yy d.zz, read_config.cpp:126), or with localvol each
+    // atom's own (:266-283), as sld_download gives it
     for (int i = 0; i < a.n; ++i) {
         const double* r = &a.r[3 * i];
         const double* p = &a.p[3 * i];
@@ -199,7 +201,7 @@ void write_config(sld_ctx* ctx, const std::string& body, int n_output, Atoms& a,
                      i, a.el[i].c_str(), r[0], r[1], r[2], p[0], p[1], p[2], fo[0], fo[1], fo[2], st[0], st[1], st[2],
                      st[3], st[4],
                      st[5], a.rho[i], a.ke[i], a.pe[i], s[0], s[1], s[2], s0, hh[0], hh[0], hh[2], hl[0], hl[0],
-                     hl[2], a.me[i], a.me0[i], lv);
+                     hl[2], a.me[i], a.me0[i], a.lv[i]);
     }
     std::fclose(f);
     if (!eltemp) return;
@@ -298,6 +300,7 @@ void write_variables_all(const Vars& v, const sld_params& prm, const sld_barosta
     out << "rcut_mag " << prm.rcut_mag << '\n';
     out << "rcut_max " << prm.rcut_max << '\n';
     out << "min_length_link_cell " << prm.min_length_link_cell << '\n';
+    if (prm.localvol) out << "rcut_vol " << prm.rcut_vol << '\n';  // read_variables.cpp:398-401
     if (prm.changestep) {
         out << "displace_limit " << prm.displace_limit << '\n';
         out << "phi_limit " << prm.phi_limit << '\n';
@@ -344,6 +347,9 @@ int main(int argc, char** argv) {
     prm.quantumnoise = (int)v.i("quantumnoise");
     prm.msteps_quantum = (int)v.i("Msteps_quantum");
     prm.nfrequency_quantum = (int)v.i("Nfrequency_quantum");
+    prm.localvol = (int)v.i("localvol");
+    prm.rcut_vol = v.d("rcut_vol");
+    prm.magmom = (int)v.i("magmom");
     sld_barostat baro = {};
     baro.stress_on = (int)v.i("STRESS");
     baro.pressure_on = (int)v.i("PRESSURE");

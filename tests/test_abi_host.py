@@ -24,13 +24,13 @@ def test_library_exports_every_declared_symbol():
                          text=True, check=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
     assert set(names) <= exported
-    assert lib.sld_version() == 1
+    assert lib.sld_version() == 2
 
 
 def test_ctypes_structs_match_header_sizes():
     """Field layout of the plain-data structs: sizes fixed by the header."""
-    assert C.sizeof(abi.SldParams) == 4 * 2 + 8 * 6 + 8 * 11 + 8 + 4 * 6 + 4 * 8 + 4 * 4
-    assert C.sizeof(abi.SldAtoms) == 14 * 8
+    assert C.sizeof(abi.SldParams) == 4 * 2 + 8 * 6 + 8 * 11 + 8 + 4 * 6 + 4 * 8 + 4 * 4 + 8
+    assert C.sizeof(abi.SldAtoms) == 15 * 8
     assert C.sizeof(abi.SldObs) == 8 + 8 + 8 * 5 + 8 * 4 + 8 * 8 + 8 + 8 * 6 + 8
 
 

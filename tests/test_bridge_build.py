@@ -7,7 +7,8 @@ prototype SPILADY declares (prototype_CPU.h:120-289, or the GPU lifecycle
 block of prototype_CPU.h:275-289 that spilady_hip_bridge.h declares for
 HIPLIB). Every optional switch the library supports is built: the shipped
 control.h, the external field and force, the two barostats, the quantum
-thermostat and readTe. Skipped where the reference's sources are absent."""
+thermostat, readTe, and the local volumes with the moments as the spin
+variable. Skipped where the reference's sources are absent."""
 import os
 import subprocess
 
@@ -33,6 +34,7 @@ VARIANTS = {
     "barostats": (["-DSTRESS", "-DPRESSURE"], ["scale_stress()", "scale_pressure()"]),
     "quantum": (["-Dquantumnoise"], ["initial_quantum_noise()", "free_quantum_noise()"]),
     "readTe": (["-DreadTe"], []),
+    "localvol_magmom": (["-Dlocalvol", "-Dmagmom"], []),
 }
 
 

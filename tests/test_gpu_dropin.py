@@ -25,6 +25,8 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "spilady_amd", "spilady_hip")
 DROPIN = os.path.join(ROOT, "oracle", "_ref", "hiplib", "spilady")
+# the same program built with -Dlocalvol -Dmagmom
+DROPIN_LVMM = os.path.join(ROOT, "oracle", "_ref", "hiplib_lvmm", "spilady")
 
 
 @pytest.mark.skipif(not os.path.exists(DROPIN), reason="the SPILADY drop-in build is absent "
@@ -76,4 +78,61 @@ def test_spilady_main_on_the_library_writes_what_the_driver_writes(tmp_path):
     enr = io.read_log(str(dirs["dropin"] / "enr-dropin.dat"))
     assert enr.shape[0] >= 4
     assert len(set(np.diff(enr[:, 1]).round(25))) > 1
+    shutil.rmtree(str(tmp_path), ignore_errors=True)
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN_LVMM), reason="the SPILADY localvol/magmom drop-in build is absent "
+                    "(oracle/build_hiplib.sh needs the reference's sources)")
+def test_spilady_main_with_localvol_and_magmom_writes_what_the_driver_writes(tmp_path):
+    """The same program built with -Dlocalvol -Dmagmom against the driver with
+    `localvol 1`, `rcut_vol 2.7` and `magmom 1`: the con- file's spin columns
+    hold the moments (read_config.cpp:96-99) and its last column the local
+    volumes, in both runs, byte for byte."""
+    from spilady_amd import constants as K
+    v, p, at = fe_params(8, 300.0, 33, thermostats=True, eltemp=True, localcolmot=True, renormalize=True,
+                         displace=0.04)
+    at.s = at.s * -K.EL_G
+    p.magmom = 1
+    p.localvol = 1
+    p.rcut_vol = 2.7
+    st = consistent_state(p, at)
+    state = io.AtomState(box=np.asarray(at.box), r=st["r"], p=st["p"], f=st["f"], s=st["s"],
+                         stress=st["stress"], rho=st["rho"], ke=st["ke"], pe=st["pe"], me=st["me"],
+                         me0=st["me0"])
+    # the shipped control.h's switches (its time loop with changestep and
+    # renormalizeEnergy among them), three steps
+    v = dict(v, in_config="con-start.dat", out_body="lvmm", runstep=0, changestep=1,
+             total_production_time=3.1e-15, displace_limit=0.01, phi_limit=0.628319, interval_of_print_out=1,
+             interval_of_config_out=2, renormalizeEnergy=1, writevsim=1, interval_of_vsim=3, vsim_prec=6,
+             localvol=1, rcut_vol=2.7, magmom=1)
+    dirs = {}
+    for name in ("driver", "dropin"):
+        d = tmp_path / name
+        d.mkdir()
+        io.write_con(str(d / "con-start.dat"), state)
+        io.write_variables(str(d / "variables.in"), v)
+        dirs[name] = d
+    before = set(os.listdir(dirs["driver"]))
+    for name, exe, args in (("driver", DRIVER, ["variables.in"]), ("dropin", DROPIN_LVMM, [])):
+        print(f"running {name}", flush=True)
+        res = subprocess.run([exe] + args, cwd=str(dirs[name]), capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, f"{name}: {res.stdout[-2000:]}\n{res.stderr[-2000:]}"
+    made = {n: set(os.listdir(d)) - before for n, d in dirs.items()}
+    cons = sorted(f for f in made["dropin"] if f.startswith("con-lvmm_"))
+    assert cons, sorted(made["dropin"])
+    # the V_sim spin file holds |m| and the moments' angles (write_vsim.cpp:107-116)
+    assert any(f.startswith("vsm-lvmm_") and f.endswith(".spin") for f in made["dropin"])
+    common = sorted((made["driver"] & made["dropin"]) - {"variables.all"})
+    assert set(cons) <= set(common), (sorted(made["driver"]), sorted(made["dropin"]))
+    differ = [f for f in common if (dirs["driver"] / f).read_bytes() != (dirs["dropin"] / f).read_bytes()]
+    only = sorted((made["dropin"] ^ made["driver"]) - {"variables.all"})
+    assert not differ, f"files that differ: {differ}; written by one side only: {only}"
+    assert not only, only
+    # the last con- file: moments of Fe's magnitude, local volumes that differ
+    # from atom to atom and fill the box
+    last = io.read_con(str(dirs["dropin"] / cons[-1]))
+    m0 = np.linalg.norm(last.s, axis=1)
+    assert 1.0 < m0.mean() < 4.0, m0.mean()
+    assert np.ptp(last.local_volume) > 0.01 * last.local_volume.mean()
+    np.testing.assert_allclose(last.local_volume.sum(), np.prod(np.asarray(at.box)[[0, 2, 5]]), rtol=1e-9)
     shutil.rmtree(str(tmp_path), ignore_errors=True)

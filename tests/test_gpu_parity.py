@@ -280,6 +280,67 @@ def test_quantum_noise_matches_oracle():
     eng.close(); orc.close()
 
 
+def test_local_volume_matches_oracle():
+    """localvol (calculate_force_energy_CPU.cpp:80-90,266-293): per-atom
+    volumes from the sums of 1/r and 1/r^2 within rcut_vol (2.7 A, between the
+    first and second shells), scaled to fill the box, and the stress over
+    them; displaced atoms so the volumes differ. The initial evaluation, then
+    6 steps with thermostats and the electron temperature: the local volumes,
+    the stress, the str- averages and the state at the tolerance."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(8, 600.0, 9, thermostats=True, eltemp=True, localcolmot=True, tilt=15.0,
+                         displace=0.05)
+    p.localvol = 1
+    p.rcut_vol = 2.7
+    st = consistent_state(p, at)
+    eng = Engine(p, st, compute_initial_forces=True)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=True, rng=orcpy.RNG_PHILOX)
+    a, b = orc.download(), eng.download()
+    assert np.ptp(a["local_volume"]) > 0.01 * a["local_volume"].mean()
+    assert_close(a["local_volume"], b["local_volume"], "localvol:initial volume")
+    assert_close(a["stress"], b["stress"], "localvol:initial stress")
+    oa = orc.step(0, 6)
+    ob = eng.step(0, 6)
+    for x, y in zip(oa, ob):
+        for k in ("ave_pe", "ave_ke", "pressure"):
+            assert abs(x[k] - y[k]) <= TOL * max(abs(x[k]), 1e-300), (x["step"], k, x[k], y[k])
+        assert relerr(x["ave_stress"], y["ave_stress"]) <= TOL, x["step"]
+    _compare(eng, orc, "localvol", fields=ATOM_FIELDS + ("local_volume",), cells=True)
+    eng.close(); orc.close()
+
+
+def test_magmom_matches_oracle():
+    """magmom (control.h:36): the spin column holds the moments m, the spins
+    are s = m / (-g) inside each spin pass and m = -g s after it
+    (core_ds_CPU.cpp:37-81), with the exchange and Landau tables scaled by
+    powers of g (potential_CPU.cpp:125-160). The initial evaluation and 6
+    steps with thermostats and the electron temperature, then new moments
+    uploaded into both and 3 more steps: the moments and the state at the
+    tolerance."""
+    from spilady_amd.engine import Engine
+    v, p, at = fe_params(8, 600.0, 21, thermostats=True, eltemp=True, localcolmot=True)
+    at.s = at.s * -K.EL_G
+    p.magmom = 1
+    st = consistent_state(p, at)
+    np.testing.assert_array_equal(st["s"], at.s)   # the column comes back as moments
+    eng = Engine(p, st, compute_initial_forces=True)
+    orc = orcpy.Oracle(p, st, compute_initial_forces=True, rng=orcpy.RNG_PHILOX)
+    _compare(eng, orc, "magmom:initial")
+    orc.step(0, 6)
+    eng.step(0, 6)
+    _compare(eng, orc, "magmom:6 steps", cells=True)
+    # a moment field rotated by a fixed random rotation, uploaded into both
+    b = orc.download()
+    q, _ = np.linalg.qr(np.random.default_rng(5).standard_normal((3, 3)))
+    b["s"] = b["s"] @ q
+    orc.upload(b)
+    eng.upload(b)
+    orc.step(6, 3)
+    eng.step(6, 3)
+    _compare(eng, orc, "magmom:after upload", cells=True)
+    eng.close(); orc.close()
+
+
 def test_external_field_and_force_match_oracle():
     """extfield and extforce: random per-atom fields (about 10 T) and forces
     (0.1 eV/A), 8 steps with thermostats and the electron temperature; the

@@ -14,6 +14,7 @@ import pytest
 
 from oracle import orcpy, refpy
 from spilady_amd import abi, configs
+from spilady_amd import constants as K
 from spilady_amd.io import AtomState
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -379,3 +380,85 @@ def test_quantum_noise_matches_reference_fixture():
     o.close()
     for k in ("r", "p", "s", "f"):
         np.testing.assert_array_equal(b[k], g[f"final_{k}"], err_msg=k)
+
+
+@pytest.mark.skipif(not refpy.available_lv(),
+                    reason="reference with localvol not built (oracle/Makefile.ref OUT=_ref/lv)")
+def test_live_reference_local_volume():
+    """localvol (calculate_force_energy_CPU.cpp:80-90,266-293): each atom's
+    volume from its sums of 1/r and 1/r^2 within rcut_vol, scaled so the
+    volumes fill the box, and the per-atom stress over it. On a lattice with
+    displaced atoms (volumes that differ from atom to atom) and rcut_vol
+    between the first and second shells, 4 steps of core() with thermostats
+    and the electron temperature: stress, local volumes and state bit for bit."""
+    at = configs.make_bcc_fe(8, 600.0, 9, tilt_max_deg=15.0, displace_sigma=0.05)
+    v = configs.fe_variables(600.0, 9, thermostats=True, eltemp=True, localcolmot=True, renormalize=False)
+    v["rcut_vol"] = 2.7
+    st = refpy.consistent_start(v, at)
+    d = tempfile.mkdtemp()
+    ref = refpy.Reference(refpy.REF_LIB_LV)
+    ref.init_memory(d, v, st)
+    for cur in range(4):
+        ref.run(cur, 1, False)
+    a = ref.atoms()
+    lv_ref = ref.local_volume()
+    ref.free()
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), st.box, st.natom)
+    p.renormalize_energy = 0
+    p.localvol = 1
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    for cur in range(4):
+        o.core(cur)
+    b = o.download()
+    o.close()
+    assert np.ptp(lv_ref) > 0.01 * lv_ref.mean()   # the volumes do differ between atoms
+    # the volume factor makes them fill the box (box_volume = d.xx d.yy d.zz)
+    np.testing.assert_allclose(lv_ref.sum(), np.prod(np.asarray(st.box)[[0, 2, 5]]), rtol=1e-12)
+    np.testing.assert_array_equal(b["local_volume"], lv_ref)
+    for k in ("r", "p", "f", "s", "rho", "pe", "me", "me0", "stress"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.skipif(not refpy.available_mm(), reason="reference magmom build absent")
+def test_live_reference_magmom():
+    """magmom (control.h:36): the spin column carries the moments m; the spins
+    follow as s = m / (-g) at the start of each spin pass and m = -g s after it
+    (core_ds_CPU.cpp:37-81, read_config.cpp:136-139), and the exchange and
+    Landau tables carry g^2, g^4, g^6, g^8 (potential_CPU.cpp:125-160). 4 steps
+    of core() with thermostats and the electron temperature: moments and
+    state bit for bit, and a trajectory that differs from the same start read
+    as spins."""
+    at = configs.make_bcc_fe(8, 600.0, 9, tilt_max_deg=15.0, displace_sigma=0.05)
+    at.s = at.s * -K.EL_G   # the moments of the same magnetic state
+    v = configs.fe_variables(600.0, 9, thermostats=True, eltemp=True, localcolmot=True, renormalize=False)
+    st = refpy.consistent_start(v, at, lib=refpy.REF_LIB_MM)
+    d = tempfile.mkdtemp()
+    ref = refpy.Reference(refpy.REF_LIB_MM)
+    ref.init_memory(d, v, st)
+    for cur in range(4):
+        ref.run(cur, 1, False)
+    a = ref.atoms()
+    m_ref = ref.moments()
+    ref.free()
+    p = abi.params_from_variables(dict(v, **refpy.REF_SWITCHES), st.box, st.natom)
+    p.renormalize_energy = 0
+    p.magmom = 1
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    for cur in range(4):
+        o.core(cur)
+    b = o.download()
+    o.close()
+    # the moments are -g times the spins the reference keeps beside them
+    np.testing.assert_array_equal(m_ref, -K.EL_G * a["s"])
+    np.testing.assert_array_equal(b["s"], m_ref)
+    for k in ("r", "p", "f", "rho", "pe", "me", "me0", "stress"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    # the same column read as spins: the same energies (J g^2 |m/g|^2 = J |m|^2)
+    # but precession g times slower, so another trajectory
+    p.magmom = 0
+    o = orcpy.Oracle(p, st, rng=orcpy.RNG_MWC)
+    for cur in range(4):
+        o.core(cur)
+    c = o.download()
+    o.close()
+    assert np.abs(c["s"] - b["s"]).max() > 1e-3 * np.abs(b["s"]).max()
