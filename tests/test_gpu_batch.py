@@ -97,3 +97,36 @@ def test_batch_quantum_noise_matches_oracle():
             assert_close(x[f], y[f], f"replica {r} {f}", TOL)
         o.close()
     b.close()
+
+
+def test_batch_local_volume_and_moments_equal_each_replica_alone():
+    """localvol and magmom in a batch: each replica's volumes are normalised
+    to its own box (one per-replica sum), and its moments convert to spins and
+    back on its own slots; every replica equals its single-system run bit for
+    bit, the local volumes included."""
+    from spilady_amd import constants as K
+    from spilady_amd.engine import BatchEngine, Engine
+    reps = []
+    for r, (nu, T, seed) in enumerate(REPLICAS):
+        v, p, at = fe_params(nu, T, seed, thermostats=True, replica=r, displace=0.04)
+        at.s = at.s * -K.EL_G
+        p.localvol = 1
+        p.rcut_vol = 2.7
+        p.magmom = 1
+        reps.append((p, consistent_state(p, at)))
+    nsteps = 6
+    b = BatchEngine([p for p, _ in reps], [st for _, st in reps])
+    obs_b = b.step(0, nsteps)
+    for r, (p, st) in enumerate(reps):
+        e = Engine(p, st)
+        obs_s = e.step(0, nsteps)
+        for k in range(nsteps):
+            assert _obs_equal(obs_b[k][r], obs_s[k]), (r, k)
+        b.select(r)
+        a1, a2 = b.download(), e.download()
+        for f in FIELDS + ("local_volume",):
+            np.testing.assert_array_equal(a1[f], a2[f], err_msg=f"replica {r} {f}")
+        box = np.asarray(p.box[:])
+        np.testing.assert_allclose(a1["local_volume"].sum(), box[0] * box[2] * box[5], rtol=1e-12)
+        e.close()
+    b.close()
