@@ -108,3 +108,40 @@ def test_production_size_graph_replay_is_bitwise_eager(production):
     bad = [f"graph vs eager: {m}" for m in _diffs(eager, graph)]
     bad += [f"second eager vs eager: {m}" for m in _diffs(eager, eager2)]
     assert not bad, "\n".join(bad)
+
+
+def test_production_size_local_volume_and_moments(production):
+    """The production configuration with localvol (rcut_vol 2.7 A, between the
+    first and second shells) and magmom on: the local volumes fill the box,
+    every moment comes back as -g times its spin (each spin pass ends with
+    m = -g s, core_ds_CPU.cpp:73-81), and the stress average of the last
+    printed step equals the mean of the per-atom stress, in GPa."""
+    from spilady_amd import constants as K
+    from spilady_amd.engine import Engine
+    at, p = production
+    # moments -s: the spins s = m / (-g) are then 1/g of the plain run's, and
+    # the g^2-scaled exchange (J g^2 |s|^2) keeps the plain run's energies.
+    # Moments -g s put the Landau terms far from their minimum: at this size
+    # the oracle's state turns non-finite in the second step.
+    at = at.copy()
+    at.s = -at.s
+    p.localvol = 1
+    p.rcut_vol = 2.7
+    p.magmom = 1
+    try:
+        e = Engine(p, at, compute_initial_forces=True)
+        obs = e.step(0, 2)
+        a = e.download()
+        e.close()
+    finally:
+        p.localvol = 0
+        p.magmom = 0
+    lv = a["local_volume"]
+    box = np.asarray(at.box)
+    assert np.all(np.isfinite(lv)) and lv.min() > 0.0
+    assert lv.sum() == pytest.approx(box[0] * box[2] * box[5], rel=1e-12)
+    m0 = np.linalg.norm(a["s"], axis=1)
+    assert np.all(np.isfinite(m0)) and 0.3 < m0.mean() < 3.0
+    # str- columns in GPa (check_stress_CPU.cpp:51-56)
+    np.testing.assert_allclose(obs[-1]["ave_stress"], a["stress"].mean(axis=0) * 160.217653, rtol=1e-10,
+                               atol=1e-12)
