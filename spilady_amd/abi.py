@@ -12,6 +12,9 @@ from . import constants as K
 _dp = C.POINTER(C.c_double)
 
 
+# SLD_ABI_VERSION of include/spilady_hip.h, whose struct layouts this module mirrors
+ABI_VERSION = 2
+
 class SldParams(C.Structure):
     _fields_ = [
         ("natom", C.c_int32),

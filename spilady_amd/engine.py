@@ -83,6 +83,10 @@ def load(path: Optional[str] = None):
     A = C.POINTER(abi.SldAtoms)
     V = C.c_void_p
     L.sld_version.restype = C.c_int
+    if L.sld_version() != abi.ABI_VERSION:
+        # the structs below would be read with another layout
+        raise RuntimeError(f"{p} has ABI version {L.sld_version()}, this package expects {abi.ABI_VERSION}: "
+                           "rebuild it (make -C spilady_amd/csrc)")
     L.sld_plan.argtypes = [_dp, C.c_double, _ip, _ip, _ip, _ip, _ip, _ip]
     L.sld_device_info.argtypes = [C.c_char_p, C.c_int, _ip]
     L.sld_create.argtypes = [P, A, C.c_int, C.POINTER(V)]

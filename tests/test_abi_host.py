@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
                          text=True, check=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
     assert set(names) <= exported
-    assert lib.sld_version() == 2
+    assert lib.sld_version() == abi.ABI_VERSION == 2
 
 
 def test_ctypes_structs_match_header_sizes():
